@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the pure-gauge HMC trajectory on one MI355X.
+
+Workload: SU(2), d = 4, 32^4 periodic, beta = 2.4, Omelyan OMF2 with 20 steps,
+tau = 1, seeded hot start (Haar-uniform links), Gaussian momenta (device Philox
+stream), Metropolis accept/reject.  A step is one HMC trajectory: momentum
+refresh, H_old, 20 OMF2 steps (41 force evaluations, 40 link updates), final
+restoreSU, H_new, accept/reject.  One trajectory performs 4 V * 20
+link-updates.
+
+`value` is the FP32 form's link-updates/s (links, staples, forces and momenta in
+FP32, compensated staple sums, reunitarisation every step, FP64 action /
+kinetic sums and accept/reject).  The plain FP64 form is timed the same way and
+reported under "fp64", with the FP32-vs-FP64 dH differences from identical
+starts and momenta.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def kernel_bytes_per_link(prec: int) -> int:
+    # fused force + momentum + link update: read U, read P, write P, write U
+    r = 8 if prec == 0 else 4
+    return r * (4 + 3 + 3 + 4)
+
+
+def run_form(S, prec, L, nsteps, steps, warmup, seed, beta):
+    dims = (L, L, L, L, 4)
+    lat = S.Lattice(S.SU2, dims, beta, precision=prec)
+    lat.haarstart(seed)
+    V = lat.V
+    for i in range(warmup):
+        lat.md_update(seed + 1000 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+    lat.synchronize()
+    acc = 0
+    dhs = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r = lat.md_update(seed + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+        acc += r["accept"]
+        dhs.append(r["deltaH"])
+    lat.synchronize()
+    dt = time.perf_counter() - t0
+    # kernel timing pass (HIP events on the handle's stream), outside the timed region
+    lat.profile(True)
+    lat.profile_reset()
+    nprof = 2
+    for i in range(nprof):
+        lat.md_update(seed + 5000 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+    ms_fused, n_fused = lat.profile_read("md_fused")
+    kinds = {}
+    for k in ("md_fused", "md_p_only", "plaq", "sumsq", "reduce", "momenta", "update_gauge"):
+        ms, n = lat.profile_read(k)
+        if n:
+            kinds[k] = {"ms_total": round(ms, 4), "launches": n, "us_avg": round(1000 * ms / n, 2)}
+    lat.profile(False)
+    lat.close()
+    link_updates = 4 * V * nsteps * steps
+    avg_us = 1000.0 * ms_fused / max(n_fused, 1)
+    bytes_launch = kernel_bytes_per_link(prec) * 4 * V
+    achieved = bytes_launch / (avg_us * 1e-6) / 1e9
+    return {
+        "link_updates_per_s": link_updates / dt,
+        "ms_per_traj": 1000.0 * dt / steps,
+        "acceptance": acc / steps,
+        "mean_abs_dH": sum(abs(x) for x in dhs) / len(dhs),
+        "fused_avg_us": avg_us,
+        "fused_bytes_per_launch": bytes_launch,
+        "fused_achieved_GBs": achieved,
+        "kernels": kinds,
+    }
+
+
+def delta_dH(S, L, nsteps, beta, seed, ntraj):
+    """FP32 vs FP64 from identical start and momenta, and FP32 reversibility."""
+    dims = (L, L, L, L, 4)
+    out = []
+    l64 = S.Lattice(S.SU2, dims, beta, precision=S.FP64)
+    l32 = S.Lattice(S.SU2, dims, beta, precision=S.FP32)
+    l64.haarstart(seed)
+    # thermalise a little in FP64 so the configuration is not a hot start
+    for i in range(5):
+        l64.md_update(seed + 100 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+    for i in range(ntraj):
+        l32.set_links(l64.get_links())
+        r64 = l64.md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE, revtest=True)
+        r32 = l32.md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE, revtest=True)
+        out.append((r32["deltaH"] - r64["deltaH"], r32["deltadeltaH"], r64["deltadeltaH"],
+                    r32["accept"] == r64["accept"]))
+    l64.close()
+    l32.close()
+    return {
+        "ntraj": ntraj,
+        "max_abs_dH32_minus_dH64": max(abs(o[0]) for o in out),
+        "max_abs_ddH_fp32": max(abs(o[1]) for o in out),
+        "max_abs_ddH_fp64": max(abs(o[2]) for o in out),
+        "same_accept_decisions": sum(o[3] for o in out),
+    }
+
+
+def cpu_baseline(L, nsteps, beta, seed):
+    """The reference's own md_update (oracle/_ref build of its hot-path headers),
+    one OMF2 trajectory at the benchmark size on all host cores; falls back to
+    the C oracle when the reference build is absent."""
+    import numpy as np
+    from oracle import oracle as orc
+    dims = (L, L, L, L, 4)
+    ncores = os.cpu_count() or 1
+    try:
+        import psutil
+        ncores = len(psutil.Process().cpu_affinity())
+    except Exception:
+        pass
+    ncores = min(ncores, 16)
+    os.environ["OMP_NUM_THREADS"] = str(ncores)
+    kind = "reference" if orc.have_reference() else "port"
+    lib = orc.Reference() if kind == "reference" else orc.Oracle()
+    U = orc.Oracle().haarstart(orc.SU2, dims, seed)
+    t0 = time.perf_counter()
+    lib.md_update(orc.SU2, dims, U, beta, orc.OMF2, nsteps, seed)
+    dt = time.perf_counter() - t0
+    return {"value": 4 * (L ** 4) * nsteps / dt, "unit": "link-updates/s", "cores": ncores,
+            "kind": kind,
+            "sample": f"1 OMF2({nsteps}) trajectory of md_update at {L}^4 (incl. momenta, H, copy), "
+                      f"{dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50, help="timed trajectories per form")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--nsteps", type=int, default=20)
+    ap.add_argument("--beta", type=float, default=2.4)
+    ap.add_argument("--seed", type=int, default=13526463)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--ddh-traj", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the lattice is not decomposed across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    import su2_amd as S
+    S.load_library()
+    f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
+    f64 = run_form(S, S.FP64, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
+    ddh = delta_dH(S, args.L, args.nsteps, args.beta, args.seed, args.ddh_traj)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
+    V = args.L ** 4
+    # the compulsory-bytes definition of the metric: per link per trajectory
+    # (2n+1) force sweeps F and 2n link sweeps G (FP32: F = 40 B, G = 44 B)
+    n = args.nsteps
+    traj_bytes32 = ((2 * n + 1) * 40 + 2 * n * 44) * 4 * V
+    traj_bytes64 = ((2 * n + 1) * 80 + 2 * n * 88) * 4 * V
+    out = {
+        "metric": "SU(2) 32^4 HMC link-updates/s, %HBM roofline; FP32 vs FP64 form at |dDH|<=1e-6",
+        "value": f32["link_updates_per_s"],
+        "unit": "link-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["ms_per_traj"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"SU(2) d=4 {args.L}^4 periodic, beta={args.beta}, OMF2 {n} steps, "
+                               f"tau=1, Haar hot start, device Gaussian momenta, FP32 form",
+                   "lattice": [args.L] * 4, "integrator": "omf2", "n_steps": n, "tau": 1.0},
+        "roofline": {"bound": "hbm", "achieved": f32["fused_achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": f32["fused_achieved_GBs"] / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "kernel": "k_md_fused<GSU2<float>,4,true>",
+                     "avg_us": f32["fused_avg_us"], "bytes_per_launch": f32["fused_bytes_per_launch"]},
+        "cpu_baseline": cpu,
+        "trajectory_roofline_frac_fp32": traj_bytes32 / (f32["ms_per_traj"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "fp32": f32,
+        "fp64": dict(f64, trajectory_roofline_frac=traj_bytes64 / (f64["ms_per_traj"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     fused_roofline_frac=f64["fused_achieved_GBs"] / HBM_PEAK_GBS),
+        "fp32_over_fp64": f32["link_updates_per_s"] / f64["link_updates_per_s"],
+        "delta_dH": ddh,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

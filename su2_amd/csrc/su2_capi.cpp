@@ -1,0 +1,1193 @@
+// su2_capi.cpp -- host side of the C ABI declared in include/su2_mi355x.h.
+//
+// Owns the device fields of one lattice, sequences the integrators of
+// integrator.hh as launches of the fused kernels, runs the HMC trajectory of
+// md_update.hh, and reproduces the reference's host random streams (libstdc++
+// std::mt19937 / normal_distribution / uniform_real_distribution) wherever the
+// reference draws on the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/su2_mi355x.h"
+#include "su2_launch.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HC(expr)                                                                         \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(SU2GPU_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+#define CK(expr)                  \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != SU2GPU_OK) return r_; \
+  } while (0)
+
+constexpr int NBUF = 4;
+constexpr int NROWS = 32;
+
+}  // namespace
+
+struct su2gpu_lattice {
+  int group = 1, prec = 0, nd = 4, nm = 3, ncomp = 4;
+  size_t Lx = 0, Ly = 0, Lz = 0, Lt = 0;
+  double beta = 0, xi = 1;
+  bool aniso = false;
+  su2k::Geo geo{};
+  int V = 0;
+  long nlinks = 0;
+  size_t real_bytes = 8, U_bytes = 0, P_bytes = 0;
+  void *U[NBUF] = {nullptr, nullptr, nullptr, nullptr};
+  int cur = 0;
+  void *P = nullptr, *deriv = nullptr;
+  double *partials = nullptr;
+  int row_stride = 0;
+  double *dsums = nullptr, *hsums = nullptr;
+  double *staging = nullptr;
+  size_t staging_cap = 0;
+  double *hstage = nullptr;
+  size_t hstage_cap = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // split md_update state
+  bool pending = false;
+  int pend_old = -1, pend_new = -1;
+  // profiling
+  bool prof = false;
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evpool;
+  std::map<std::string, std::pair<double, long>> prof_acc;
+};
+
+namespace {
+
+using L = su2gpu_lattice;
+
+// ---------------------------------------------------------------- helpers --
+int ensure_buf(L *h, int i) {
+  if (!h->U[i]) HC(hipMalloc(&h->U[i], h->U_bytes));
+  return SU2GPU_OK;
+}
+int free_buf(L *h, std::initializer_list<int> excl, int *out) {
+  for (int i = 0; i < NBUF; ++i) {
+    bool used = false;
+    for (int e : excl)
+      if (e == i) used = true;
+    if (!used) {
+      CK(ensure_buf(h, i));
+      *out = i;
+      return SU2GPU_OK;
+    }
+  }
+  return fail(SU2GPU_ESTATE, "no free link buffer");
+}
+int ensure_staging(L *h, size_t ndoubles) {
+  if (h->staging_cap < ndoubles) {
+    if (h->staging) HC(hipFree(h->staging));
+    h->staging = nullptr;
+    HC(hipMalloc(&h->staging, ndoubles * sizeof(double)));
+    h->staging_cap = ndoubles;
+  }
+  if (h->hstage_cap < ndoubles) {
+    if (h->hstage) HC(hipHostFree(h->hstage));
+    h->hstage = nullptr;
+    HC(hipHostMalloc(&h->hstage, ndoubles * sizeof(double), hipHostMallocDefault));
+    h->hstage_cap = ndoubles;
+  }
+  return SU2GPU_OK;
+}
+int ensure_deriv(L *h) {
+  if (!h->deriv) {
+    HC(hipMalloc(&h->deriv, h->P_bytes));
+    HC(hipMemsetAsync(h->deriv, 0, h->P_bytes, h->stream));
+  }
+  return SU2GPU_OK;
+}
+
+// HIP-event timing around one launch
+struct Prof {
+  L *h;
+  size_t idx = (size_t)-1;
+  Prof(L *hh, const char *name) : h(hh) {
+    if (!h->prof) return;
+    hipEvent_t a, b;
+    if (h->evpool.size() >= 2) {
+      a = h->evpool.back();
+      h->evpool.pop_back();
+      b = h->evpool.back();
+      h->evpool.pop_back();
+    } else {
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+    }
+    hipEventRecord(a, h->stream);
+    h->recs.push_back({name, a, b});
+    idx = h->recs.size() - 1;
+  }
+  ~Prof() {
+    if (idx != (size_t)-1) hipEventRecord(h->recs[idx].b, h->stream);
+  }
+};
+
+int collect_prof(L *h) {
+  if (h->recs.empty()) return SU2GPU_OK;
+  HC(hipStreamSynchronize(h->stream));
+  for (auto &r : h->recs) {
+    float ms = 0;
+    HC(hipEventElapsedTime(&ms, r.a, r.b));
+    auto &acc = h->prof_acc[r.name];
+    acc.first += ms;
+    acc.second += 1;
+    h->evpool.push_back(r.a);
+    h->evpool.push_back(r.b);
+  }
+  h->recs.clear();
+  return SU2GPU_OK;
+}
+
+int last_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SU2GPU_EHIP, "launch %s: %s", what, hipGetErrorString(e));
+  return SU2GPU_OK;
+}
+
+double Nc(const L *h) { return h->group == 1 ? 2.0 : 1.0; }
+
+su2k::FusedCoef base_coef(const L *h) {
+  su2k::FusedCoef c{};
+  c.coef = 1.0 * h->beta / Nc(h);
+  c.w_t = h->aniso ? 1.0 / h->xi : 1.0;
+  c.w_s = h->aniso ? h->xi : 1.0;
+  c.flags = (h->aniso ? su2k::F_ANISO : 0) | (h->prec == 1 ? su2k::F_COMPENSATED : 0);
+  return c;
+}
+
+double *row(L *h, int r) { return h->partials + (size_t)r * h->row_stride; }
+
+// fused "P then U" (Uout >= 0) or momentum-only (Uout < 0) step
+int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row) {
+  su2k::FusedLaunch fl;
+  fl.Uin = h->U[uin];
+  fl.Uout = uout >= 0 ? h->U[uout] : nullptr;
+  fl.P = P;
+  fl.geo = h->geo;
+  fl.c = c;
+  fl.partials = nullptr;
+  fl.pstride = h->row_stride;
+  if (accum_row >= 0) {
+    fl.c.flags |= su2k::F_ACCUM;
+    fl.partials = row(h, accum_row);
+  }
+  {
+    Prof pr(h, uout >= 0 ? "md_fused" : "md_p_only");
+    su2k::launch_fused(h->group, h->prec, fl, h->stream);
+  }
+  return last_launch("md_fused");
+}
+
+int reduce_row(L *h, int r, int nb) {
+  {
+    Prof pr(h, "reduce");
+    su2k::launch_reduce(row(h, r), nb, 1, h->dsums + 2 * r, h->stream);
+  }
+  return last_launch("reduce");
+}
+int fetch_sums(L *h) {
+  HC(hipMemcpyAsync(h->hsums, h->dsums, 2 * NROWS * sizeof(double), hipMemcpyDeviceToHost,
+                    h->stream));
+  HC(hipStreamSynchronize(h->stream));
+  return SU2GPU_OK;
+}
+long double sum_of(const L *h, int r) {
+  return (long double)h->hsums[2 * r] + (long double)h->hsums[2 * r + 1];
+}
+
+int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, int flags, int r) {
+  {
+    Prof pr(h, "plaq");
+    su2k::launch_plaq(h->group, h->prec, h->U[uin], uout >= 0 ? h->U[uout] : nullptr, h->geo,
+                      w_t, w_s, mu_start, flags, row(h, r), h->stream);
+  }
+  CK(last_launch("plaq"));
+  return reduce_row(h, r, su2k::fused_blocks(h->geo));
+}
+
+int sumsq_launch(L *h, const void *P, int r) {
+  const long n = h->nlinks * h->nm;
+  {
+    Prof pr(h, "sumsq");
+    su2k::launch_sumsq(h->prec, P, n, row(h, r), h->stream);
+  }
+  CK(last_launch("sumsq"));
+  return reduce_row(h, r, su2k::sumsq_blocks(n));
+}
+
+// ------------------------------------------------------------- integrators --
+struct Op {
+  char kind;  // 'P' or 'U'
+  double c;   // step = c * dtau
+};
+
+// the operation sequences of integrator.hh (leapfrog :26-50, omf2 :83-115,
+// omf4 :118-155 with its loop from i = 1, euler :201-217, ruth :220-241)
+int integrator_ops(int kind, size_t n, std::vector<Op> &ops) {
+  ops.clear();
+  if (n < 1) return fail(SU2GPU_EINVAL, "nsteps must be >= 1");
+  switch (kind) {
+    case SU2GPU_LEAPFROG:
+    case SU2GPU_LP_LEAPFROG:
+      ops.push_back({'P', 0.5});
+      ops.push_back({'U', 1.0});
+      for (size_t i = 0; i + 1 < n; ++i) {
+        ops.push_back({'P', 1.0});
+        ops.push_back({'U', 1.0});
+      }
+      ops.push_back({'P', 0.5});
+      break;
+    case SU2GPU_OMF2: {
+      const double lambda = 0.1938, om = 1. - 2. * lambda;
+      ops.push_back({'P', lambda});
+      for (size_t i = 0; i + 1 < n; ++i) {
+        ops.push_back({'U', 0.5});
+        ops.push_back({'P', om});
+        ops.push_back({'U', 0.5});
+        ops.push_back({'P', 2 * lambda});
+      }
+      ops.push_back({'U', 0.5});
+      ops.push_back({'P', om});
+      ops.push_back({'U', 0.5});
+      ops.push_back({'P', lambda});
+      break;
+    }
+    case SU2GPU_OMF4:
+    case SU2GPU_LP_OMF4: {
+      const double rho = 0.2539785108410595, theta = -0.03230286765269967,
+                   vartheta = 0.08398315262876693, lambda = 0.6822365335719091;
+      const double e[10] = {rho, lambda, theta, 0.5 * (1 - 2. * (lambda + vartheta)),
+                            (1 - 2. * (theta + rho)), 0.5 * (1 - 2. * (lambda + vartheta)),
+                            theta, lambda, rho, 2 * vartheta};
+      ops.push_back({'P', 0.5 * e[9]});
+      for (size_t i = 1; i + 1 < n; ++i)
+        for (int j = 0; j < 5; ++j) {
+          ops.push_back({'U', e[2 * j]});
+          ops.push_back({'P', e[2 * j + 1]});
+        }
+      for (int j = 0; j < 4; ++j) {
+        ops.push_back({'U', e[2 * j]});
+        ops.push_back({'P', e[2 * j + 1]});
+      }
+      ops.push_back({'U', e[8]});
+      ops.push_back({'P', 0.5 * e[9]});
+      break;
+    }
+    case SU2GPU_EULER:
+      for (size_t i = 0; i < n; ++i) {
+        ops.push_back({'P', 1.0});
+        ops.push_back({'U', 1.0});
+      }
+      break;
+    case SU2GPU_RUTH:
+      for (size_t i = 0; i < n; ++i) {
+        ops.push_back({'P', 1.0});
+        ops.push_back({'U', -1. / 24.});
+        ops.push_back({'P', -2. / 3.});
+        ops.push_back({'U', 3. / 4.});
+        ops.push_back({'P', 2. / 3.});
+        ops.push_back({'U', 7. / 24.});
+      }
+      break;
+    default:
+      // set_integrator falls back to leapfrog for an unknown id (integrator.hh:274-277)
+      return integrator_ops(SU2GPU_LEAPFROG, n, ops);
+  }
+  return SU2GPU_OK;
+}
+
+// The coefficients are multiplied by dtau exactly as the reference does:
+// leapfrog dtau/2. and dtau, omf2 lambda*dtau etc.  The products below use
+// the same association as integrator.hh so that FP64 steps match bit for bit
+// on the host side of the computation.
+struct IntegrateOut {
+  bool first_accum = false;  // rows 0..2 hold kin_old/act/kin_new of the first step
+  bool last_p_only = false;  // rows 3..5 hold the last momentum-only step sums
+  int final_buf = -1;
+};
+
+int run_integrator(L *h, int kind, size_t nsteps, double tau, size_t exponent,
+                   std::initializer_list<int> protect, bool accum, int row_base,
+                   IntegrateOut *io) {
+  std::vector<Op> ops;
+  CK(integrator_ops(kind, nsteps, ops));
+  const double dtau = tau / double(nsteps);
+  su2k::FusedCoef c0 = base_coef(h);
+  if (kind == SU2GPU_LP_LEAPFROG || kind == SU2GPU_LP_OMF4) {
+    c0.flags |= su2k::F_ROUND;
+    c0.round_n = (double)(size_t)std::pow(10, (double)exponent);
+  }
+  if (h->prec == 1 && h->group == 1) c0.flags |= su2k::F_RESTORE;  // FP32: reunitarise every step
+  int cur = h->cur;
+  auto is_prot = [&](int b) {
+    for (int p : protect)
+      if (p == b) return true;
+    return false;
+  };
+  bool first = true;
+  size_t i = 0;
+  while (i < ops.size()) {
+    const Op &o = ops[i];
+    if (o.kind == 'P') {
+      su2k::FusedCoef c = c0;
+      c.ep = o.c * dtau;
+      const int acc_row = (accum && first) ? row_base + 0 : -1;
+      const bool last_p = (i + 1 == ops.size());
+      if (i + 1 < ops.size() && ops[i + 1].kind == 'U') {
+        c.eu = ops[i + 1].c * dtau;
+        int out;
+        CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
+                        protect.size() > 1 ? *(protect.begin() + 1) : cur},
+                    &out));
+        CK(fused(h, cur, out, h->P, c, acc_row));
+        cur = out;
+        i += 2;
+      } else {
+        const int r = (accum && first) ? row_base + 0 : (accum && last_p) ? row_base + 3 : -1;
+        CK(fused(h, cur, -1, h->P, c, r));
+        if (accum && last_p && !first) io->last_p_only = true;
+        i += 1;
+      }
+      if (first && accum) io->first_accum = true;
+      first = false;
+    } else {
+      if (is_prot(cur)) {
+        int out;
+        CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
+                        protect.size() > 1 ? *(protect.begin() + 1) : cur},
+                    &out));
+        HC(hipMemcpyAsync(h->U[out], h->U[cur], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+        cur = out;
+      }
+      {
+        Prof pr(h, "update_gauge");
+        su2k::launch_update_gauge(h->group, h->prec, h->U[cur], h->P, h->V, h->nd, o.c * dtau,
+                                  c0.round_n, c0.flags & (su2k::F_ROUND | su2k::F_RESTORE),
+                                  h->stream);
+      }
+      CK(last_launch("update_gauge"));
+      first = false;
+      i += 1;
+    }
+  }
+  io->final_buf = cur;
+  return SU2GPU_OK;
+}
+
+// host-side random start configurations with the reference's exact streams
+constexpr double ref_pi() { return 3.14159265358979311599796346854; }  // atan(1)*4
+
+void random_su2(std::mt19937 &e, double delta, double *q) {
+  // random_element (random_element.hh:9-25)
+  std::uniform_real_distribution<double> dist1(-1., 1.);
+  std::uniform_real_distribution<double> dist2(0., 2 * ref_pi());
+  std::uniform_real_distribution<double> dist3(0., delta * 2 * ref_pi());
+  const double alpha = dist3(e);
+  const double u = dist1(e);
+  const double theta = dist2(e);
+  const double r = std::sqrt(1 - u * u);
+  const double salpha = std::sin(alpha);
+  const double sr = salpha * r;
+  q[0] = std::cos(alpha);
+  q[1] = salpha * u;
+  q[2] = sr * std::sin(theta);
+  q[3] = sr * std::cos(theta);
+}
+double random_u1(std::mt19937 &e, double delta) {
+  std::uniform_real_distribution<double> dist(-ref_pi() * delta, ref_pi() * delta);
+  return dist(e);
+}
+
+// (a1 + i b1)(a2 + i b2) in std::complex order
+inline void cm(double ar, double ai, double br, double bi, double &rr, double &ri) {
+  rr = ar * br - ai * bi;
+  ri = ar * bi + ai * br;
+}
+void su2_mul_host(const double *u, const double *v, double *o) {
+  double t1r, t1i, t2r, t2i;
+  cm(u[0], u[1], v[0], v[1], t1r, t1i);
+  cm(u[2], u[3], v[2], -v[3], t2r, t2i);
+  double ar = t1r - t2r, ai = t1i - t2i;
+  cm(u[0], u[1], v[2], v[3], t1r, t1i);
+  cm(u[2], u[3], v[0], -v[1], t2r, t2i);
+  o[0] = ar;
+  o[1] = ai;
+  o[2] = t1r + t2r;
+  o[3] = t1i + t2i;
+}
+
+int upload_links_packed(L *h, const double *src_dev_staging, int stride, int offset) {
+  int b;
+  CK(ensure_buf(h, h->cur));
+  b = h->cur;
+  {
+    Prof pr(h, "convert");
+    su2k::launch_links_from_aos(h->group, h->prec, src_dev_staging, stride, offset, h->U[b], h->V,
+                                h->nd, h->stream);
+  }
+  return last_launch("links_from_aos");
+}
+
+// host AoS (doubles, per-link stride / offset) -> device links
+int set_links_host(L *h, const double *src, int stride, int offset) {
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending: call su2gpu_md_finish first");
+  const size_t n = (size_t)h->nlinks * stride;
+  CK(ensure_staging(h, n));
+  HC(hipMemcpyAsync(h->staging, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  CK(upload_links_packed(h, h->staging, stride, offset));
+  HC(hipStreamSynchronize(h->stream));
+  return SU2GPU_OK;
+}
+int get_links_host(L *h, double *dst, int stride, int offset) {
+  const size_t n = (size_t)h->nlinks * stride;
+  CK(ensure_staging(h, n));
+  if (stride != h->ncomp) HC(hipMemsetAsync(h->staging, 0, n * sizeof(double), h->stream));
+  {
+    Prof pr(h, "convert");
+    su2k::launch_links_to_aos(h->group, h->prec, h->U[h->cur], h->staging, stride, offset, h->V,
+                              h->nd, h->stream);
+  }
+  CK(last_launch("links_to_aos"));
+  HC(hipMemcpyAsync(dst, h->staging, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HC(hipStreamSynchronize(h->stream));
+  return SU2GPU_OK;
+}
+int set_mom_host(L *h, void *dstP, const double *src) {
+  const size_t n = (size_t)h->nlinks * h->nm;
+  CK(ensure_staging(h, n));
+  HC(hipMemcpyAsync(h->staging, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  {
+    Prof pr(h, "convert");
+    su2k::launch_mom_from_aos(h->prec, h->staging, dstP, h->V, h->nd, h->nm, h->stream);
+  }
+  CK(last_launch("mom_from_aos"));
+  HC(hipStreamSynchronize(h->stream));
+  return SU2GPU_OK;
+}
+int get_mom_host(L *h, const void *srcP, double *dst) {
+  const size_t n = (size_t)h->nlinks * h->nm;
+  CK(ensure_staging(h, n));
+  {
+    Prof pr(h, "convert");
+    su2k::launch_mom_to_aos(h->prec, srcP, h->staging, h->V, h->nd, h->nm, h->stream);
+  }
+  CK(last_launch("mom_to_aos"));
+  HC(hipMemcpyAsync(dst, h->staging, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HC(hipStreamSynchronize(h->stream));
+  return SU2GPU_OK;
+}
+
+// initnormal (adjointfield.hh:240-258) on the host into hstage
+void host_normals(L *h, std::mt19937 &engine) {
+  std::normal_distribution<double> normal(0., 1.);
+  const size_t n = (size_t)h->nlinks * h->nm;
+  for (size_t i = 0; i < n; ++i) h->hstage[i] = normal(engine);
+}
+
+int check(const L *h) {
+  if (!h) return fail(SU2GPU_EINVAL, "null handle");
+  return SU2GPU_OK;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+
+extern "C" {
+
+const char *su2gpu_last_error(void) { return g_err.c_str(); }
+
+int su2gpu_device_count(int *n) {
+  if (!n) return fail(SU2GPU_EINVAL, "null pointer");
+  HC(hipGetDeviceCount(n));
+  return SU2GPU_OK;
+}
+
+int su2gpu_create(su2gpu_lattice **out, int group, size_t Lx, size_t Ly, size_t Lz, size_t Lt,
+                  size_t ndims, double beta, double xi, int precision) {
+  if (!out) return fail(SU2GPU_EINVAL, "null output handle");
+  *out = nullptr;
+  if (group != SU2GPU_U1 && group != SU2GPU_SU2) return fail(SU2GPU_EINVAL, "group must be U1 or SU2");
+  if (ndims < 2 || ndims > 4) return fail(SU2GPU_EINVAL, "ndims must be 2, 3 or 4");
+  if (precision != SU2GPU_FP64 && precision != SU2GPU_FP32) return fail(SU2GPU_EINVAL, "bad precision");
+  if (!Lx || !Ly || !Lz || !Lt) return fail(SU2GPU_EINVAL, "extents must be >= 1");
+  if (ndims < 4 && Lz != 1) return fail(SU2GPU_EINVAL, "ndims < 4 needs Lz = 1");
+  if (ndims < 3 && Ly != 1) return fail(SU2GPU_EINVAL, "ndims < 3 needs Ly = 1");
+  if (!(xi > 0)) return fail(SU2GPU_EINVAL, "xi must be > 0");
+  const unsigned long long V = (unsigned long long)Lx * Ly * Lz * Lt;
+  const int nm = group == SU2GPU_SU2 ? 3 : 1;
+  if (V * ndims * (nm > 4 ? nm : 4) >= (1ull << 31))
+    return fail(SU2GPU_EINVAL, "lattice too large for 32-bit indexing");
+  int ndev = 0;
+  HC(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(SU2GPU_EHIP, "no HIP device");
+  L *h = new L();
+  h->group = group;
+  h->prec = precision;
+  h->nd = (int)ndims;
+  h->nm = nm;
+  h->ncomp = group == SU2GPU_SU2 ? 4 : 1;
+  h->Lx = Lx; h->Ly = Ly; h->Lz = Lz; h->Lt = Lt;
+  h->beta = beta;
+  h->xi = xi;
+  h->aniso = (xi != 1.0);
+  h->V = (int)V;
+  h->nlinks = (long)V * (long)ndims;
+  h->real_bytes = precision == SU2GPU_FP64 ? 8 : 4;
+  h->U_bytes = (size_t)h->nlinks * h->ncomp * h->real_bytes;
+  h->P_bytes = (size_t)h->nlinks * h->nm * h->real_bytes;
+  // direction mu = (t, x, y, z)
+  const int Ls[4] = {(int)Lt, (int)Lx, (int)Ly, (int)Lz};
+  h->geo.L[0] = Ls[0]; h->geo.L[1] = Ls[1]; h->geo.L[2] = Ls[2]; h->geo.L[3] = Ls[3];
+  h->geo.str[3] = 1;
+  h->geo.str[2] = Ls[3];
+  h->geo.str[1] = Ls[3] * Ls[2];
+  h->geo.str[0] = Ls[3] * Ls[2] * Ls[1];
+  h->geo.V = h->V;
+  h->geo.nd = h->nd;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  const int nbf = su2k::fused_blocks(h->geo), nbs = su2k::sumsq_blocks((long)h->nlinks * h->nm);
+  h->row_stride = nbf > nbs ? nbf : nbs;
+  if (e == hipSuccess) e = hipMalloc(&h->U[0], h->U_bytes);
+  if (e == hipSuccess) e = hipMalloc(&h->U[1], h->U_bytes);
+  if (e == hipSuccess) e = hipMalloc(&h->U[2], h->U_bytes);
+  if (e == hipSuccess) e = hipMalloc(&h->P, h->P_bytes);
+  if (e == hipSuccess) e = hipMalloc(&h->partials, (size_t)NROWS * h->row_stride * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&h->dsums, 2 * NROWS * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&h->hsums, 2 * NROWS * sizeof(double), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemsetAsync(h->P, 0, h->P_bytes, h->stream);
+  if (e != hipSuccess) {
+    fail(SU2GPU_EHIP, "create: %s", hipGetErrorString(e));
+    su2gpu_destroy(h);
+    return SU2GPU_EHIP;
+  }
+  *out = h;
+  int r = su2gpu_coldstart(h);
+  if (r != SU2GPU_OK) {
+    su2gpu_destroy(h);
+    *out = nullptr;
+  }
+  return r;
+}
+
+int su2gpu_destroy(su2gpu_lattice *h) {
+  if (!h) return SU2GPU_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto &r : h->recs) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto ev : h->evpool) hipEventDestroy(ev);
+  for (int i = 0; i < NBUF; ++i)
+    if (h->U[i]) hipFree(h->U[i]);
+  if (h->P) hipFree(h->P);
+  if (h->deriv) hipFree(h->deriv);
+  if (h->partials) hipFree(h->partials);
+  if (h->dsums) hipFree(h->dsums);
+  if (h->hsums) hipHostFree(h->hsums);
+  if (h->staging) hipFree(h->staging);
+  if (h->hstage) hipHostFree(h->hstage);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return SU2GPU_OK;
+}
+
+int su2gpu_set_beta(su2gpu_lattice *h, double beta) {
+  CK(check(h));
+  h->beta = beta;
+  return SU2GPU_OK;
+}
+int su2gpu_get_beta(const su2gpu_lattice *h, double *beta) {
+  CK(check(h));
+  if (!beta) return fail(SU2GPU_EINVAL, "null pointer");
+  *beta = h->beta;
+  return SU2GPU_OK;
+}
+int su2gpu_get_geometry(const su2gpu_lattice *h, size_t out[8]) {
+  CK(check(h));
+  if (!out) return fail(SU2GPU_EINVAL, "null pointer");
+  out[0] = h->Lx; out[1] = h->Ly; out[2] = h->Lz; out[3] = h->Lt;
+  out[4] = (size_t)h->nd; out[5] = (size_t)h->V; out[6] = (size_t)h->group; out[7] = (size_t)h->prec;
+  return SU2GPU_OK;
+}
+
+int su2gpu_set_links(su2gpu_lattice *h, const void *src, int layout) {
+  CK(check(h));
+  if (!src) return fail(SU2GPU_EINVAL, "null source");
+  if (layout == SU2GPU_LAYOUT_PACKED) return set_links_host(h, (const double *)src, h->ncomp, 0);
+  if (layout == SU2GPU_LAYOUT_OBJECT) return set_links_host(h, (const double *)src, h->ncomp + 1, 1);
+  return fail(SU2GPU_EINVAL, "bad layout");
+}
+
+int su2gpu_get_links(su2gpu_lattice *h, void *dst, int layout) {
+  CK(check(h));
+  if (!dst) return fail(SU2GPU_EINVAL, "null destination");
+  if (layout == SU2GPU_LAYOUT_PACKED) return get_links_host(h, (double *)dst, h->ncomp, 0);
+  if (layout == SU2GPU_LAYOUT_OBJECT) {
+    CK(get_links_host(h, (double *)dst, h->ncomp + 1, 1));
+    // the objects' leading size_t N_c (su2.hh:13, u1.hh:12)
+    const size_t nc = h->group == SU2GPU_SU2 ? 2 : 1;
+    char *p = (char *)dst;
+    for (long i = 0; i < h->nlinks; ++i)
+      std::memcpy(p + (size_t)i * (h->ncomp + 1) * sizeof(double), &nc, sizeof(size_t));
+    return SU2GPU_OK;
+  }
+  return fail(SU2GPU_EINVAL, "bad layout");
+}
+
+int su2gpu_set_momenta(su2gpu_lattice *h, const double *src) {
+  CK(check(h));
+  if (!src) return fail(SU2GPU_EINVAL, "null source");
+  return set_mom_host(h, h->P, src);
+}
+int su2gpu_get_momenta(su2gpu_lattice *h, double *dst) {
+  CK(check(h));
+  if (!dst) return fail(SU2GPU_EINVAL, "null destination");
+  return get_mom_host(h, h->P, dst);
+}
+int su2gpu_flip_momenta(su2gpu_lattice *h) {
+  CK(check(h));
+  {
+    Prof pr(h, "scale");
+    su2k::launch_scale(h->prec, h->P, h->nlinks * h->nm, -1.0, h->stream);
+  }
+  return last_launch("scale");
+}
+int su2gpu_initnormal(su2gpu_lattice *h, uint64_t seed) {
+  CK(check(h));
+  CK(ensure_staging(h, (size_t)h->nlinks * h->nm));
+  std::mt19937 engine(seed);
+  host_normals(h, engine);
+  return set_mom_host(h, h->P, h->hstage);
+}
+
+int su2gpu_hotstart(su2gpu_lattice *h, uint64_t seed, double delta) {
+  CK(check(h));
+  if (delta < 0.) delta = 0.;
+  if (delta > 1.) delta = 1.;
+  const size_t n = (size_t)h->nlinks * h->ncomp;
+  CK(ensure_staging(h, n));
+  std::mt19937 engine((int)seed);
+  for (long i = 0; i < h->nlinks; ++i) {
+    if (h->group == SU2GPU_SU2) random_su2(engine, delta, h->hstage + 4 * i);
+    else h->hstage[i] = random_u1(engine, delta);
+  }
+  return set_links_host(h, h->hstage, h->ncomp, 0);
+}
+
+int su2gpu_coldstart(su2gpu_lattice *h) {
+  CK(check(h));
+  const size_t n = (size_t)h->nlinks * h->ncomp;
+  CK(ensure_staging(h, n));
+  for (long i = 0; i < h->nlinks; ++i) {
+    if (h->group == SU2GPU_SU2) {
+      h->hstage[4 * i] = 1.0;
+      h->hstage[4 * i + 1] = h->hstage[4 * i + 2] = h->hstage[4 * i + 3] = 0.0;
+    } else {
+      h->hstage[i] = 0.0;
+    }
+  }
+  return set_links_host(h, h->hstage, h->ncomp, 0);
+}
+
+int su2gpu_haarstart(su2gpu_lattice *h, uint64_t seed) {
+  CK(check(h));
+  const size_t n = (size_t)h->nlinks * h->ncomp;
+  CK(ensure_staging(h, n));
+  std::mt19937 engine(seed);
+  std::normal_distribution<double> normal(0., 1.);
+  std::uniform_real_distribution<double> uni(-ref_pi(), ref_pi());
+  for (long i = 0; i < h->nlinks; ++i) {
+    if (h->group == SU2GPU_SU2) {
+      double q[4], r2 = 0.0;
+      for (int k = 0; k < 4; ++k) {
+        q[k] = normal(engine);
+        r2 += q[k] * q[k];
+      }
+      const double r = std::sqrt(r2);
+      for (int k = 0; k < 4; ++k) h->hstage[4 * i + k] = q[k] / r;
+    } else {
+      h->hstage[i] = uni(engine);
+    }
+  }
+  return set_links_host(h, h->hstage, h->ncomp, 0);
+}
+
+int su2gpu_random_gauge_trafo(su2gpu_lattice *h, uint64_t seed) {
+  CK(check(h));
+  // sequential over sites in the reference order (random_gauge_trafo.hh:15-31):
+  // done on the host on the AoS links, then uploaded
+  std::vector<double> U((size_t)h->nlinks * h->ncomp);
+  CK(get_links_host(h, U.data(), h->ncomp, 0));
+  std::mt19937 engine((int)seed);
+  const long Ls[4] = {(long)h->Lt, (long)h->Lx, (long)h->Ly, (long)h->Lz};
+  const int nd = h->nd;
+  long s = 0;
+  for (long t = 0; t < Ls[0]; ++t)
+    for (long x = 0; x < Ls[1]; ++x)
+      for (long y = 0; y < Ls[2]; ++y)
+        for (long z = 0; z < Ls[3]; ++z, ++s) {
+          long c[4] = {t, x, y, z};
+          double rq[4], ru = 0.0;
+          if (h->group == SU2GPU_SU2) random_su2(engine, 1.0, rq);
+          else ru = random_u1(engine, 1.0);
+          for (int mu = 0; mu < nd; ++mu) {
+            long cm_[4] = {c[0], c[1], c[2], c[3]};
+            cm_[mu] = (cm_[mu] - 1 + Ls[mu]) % Ls[mu];
+            const long sm = ((cm_[0] * Ls[1] + cm_[1]) * Ls[2] + cm_[2]) * Ls[3] + cm_[3];
+            const size_t i = (size_t)s * nd + mu, j = (size_t)sm * nd + mu;
+            if (h->group == SU2GPU_SU2) {
+              double o[4];
+              su2_mul_host(rq, &U[4 * i], o);
+              std::memcpy(&U[4 * i], o, sizeof(o));
+              const double rd[4] = {rq[0], -rq[1], -rq[2], -rq[3]};
+              su2_mul_host(&U[4 * j], rd, o);
+              std::memcpy(&U[4 * j], o, sizeof(o));
+            } else {
+              U[i] = ru + U[i];
+              U[j] = U[j] + (-ru);
+            }
+          }
+        }
+  return set_links_host(h, U.data(), h->ncomp, 0);
+}
+
+int su2gpu_restoreSU(su2gpu_lattice *h) {
+  CK(check(h));
+  {
+    Prof pr(h, "restore");
+    su2k::launch_restore(h->group, h->prec, h->U[h->cur], (int)h->nlinks, h->stream);
+  }
+  return last_launch("restore");
+}
+
+int su2gpu_retr_sum_plaquettes(su2gpu_lattice *h, double xi, int anisotropic, int spatial_only,
+                               double *res) {
+  CK(check(h));
+  if (!res) return fail(SU2GPU_EINVAL, "null pointer");
+  const int flags = anisotropic ? su2k::F_ANISO : 0;
+  CK(plaq_launch(h, h->cur, -1, anisotropic ? 1.0 / xi : 1.0, anisotropic ? xi : 1.0,
+                 spatial_only ? 1 : 0, flags, 6));
+  CK(fetch_sums(h));
+  *res = (double)sum_of(h, 6);
+  return SU2GPU_OK;
+}
+
+int su2gpu_gauge_energy(su2gpu_lattice *h, int spatial_only, double *res) {
+  return su2gpu_retr_sum_plaquettes(h, 1.0, 0, spatial_only, res);
+}
+
+int su2gpu_gauge_action(su2gpu_lattice *h, double *S) {
+  CK(check(h));
+  if (!S) return fail(SU2GPU_EINVAL, "null pointer");
+  double r;
+  CK(su2gpu_retr_sum_plaquettes(h, h->xi, h->aniso ? 1 : 0, 0, &r));
+  *S = -h->beta * r / Nc(h);
+  return SU2GPU_OK;
+}
+
+int su2gpu_kinetic_energy(su2gpu_lattice *h, double *K) {
+  CK(check(h));
+  if (!K) return fail(SU2GPU_EINVAL, "null pointer");
+  CK(sumsq_launch(h, h->P, 7));
+  CK(fetch_sums(h));
+  *K = (double)(0.5L * sum_of(h, 7));
+  return SU2GPU_OK;
+}
+
+int su2gpu_zero_derivative(su2gpu_lattice *h) {
+  CK(check(h));
+  CK(ensure_deriv(h));
+  HC(hipMemsetAsync(h->deriv, 0, h->P_bytes, h->stream));
+  return SU2GPU_OK;
+}
+
+int su2gpu_derivative_device(su2gpu_lattice *h, double fac) {
+  CK(check(h));
+  CK(ensure_deriv(h));
+  su2k::FusedCoef c = base_coef(h);
+  c.coef = fac * h->beta / Nc(h);  // fac * beta / Nc * get_deriv (flat-gaugemonomial.hh:123-124)
+  c.ep = -1.0;                     // deriv += coef * d
+  return fused(h, h->cur, -1, h->deriv, c, -1);
+}
+
+int su2gpu_get_derivative(su2gpu_lattice *h, double *deriv) {
+  CK(check(h));
+  if (!deriv) return fail(SU2GPU_EINVAL, "null destination");
+  CK(ensure_deriv(h));
+  return get_mom_host(h, h->deriv, deriv);
+}
+
+int su2gpu_derivative(su2gpu_lattice *h, double fac, double *deriv) {
+  CK(check(h));
+  if (!deriv) return fail(SU2GPU_EINVAL, "null deriv");
+  CK(ensure_deriv(h));
+  CK(set_mom_host(h, h->deriv, deriv));
+  CK(su2gpu_derivative_device(h, fac));
+  return get_mom_host(h, h->deriv, deriv);
+}
+
+int su2gpu_update_momenta(su2gpu_lattice *h, double dtau) {
+  CK(check(h));
+  su2k::FusedCoef c = base_coef(h);
+  c.ep = dtau;
+  return fused(h, h->cur, -1, h->P, c, -1);
+}
+
+int su2gpu_update_gauge(su2gpu_lattice *h, double dtau) {
+  CK(check(h));
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
+  const int flags = (h->prec == 1 && h->group == 1) ? su2k::F_RESTORE : 0;
+  {
+    Prof pr(h, "update_gauge");
+    su2k::launch_update_gauge(h->group, h->prec, h->U[h->cur], h->P, h->V, h->nd, dtau, 0.0, flags,
+                              h->stream);
+  }
+  return last_launch("update_gauge");
+}
+
+int su2gpu_integrate(su2gpu_lattice *h, int integrator, size_t nsteps, double tau, int restore,
+                     size_t exponent) {
+  CK(check(h));
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
+  IntegrateOut io;
+  CK(run_integrator(h, integrator, nsteps, tau, exponent, {}, false, 0, &io));
+  h->cur = io.final_buf;
+  if (restore) CK(su2gpu_restoreSU(h));
+  return SU2GPU_OK;
+}
+
+// ---------------------------------------------------------------- md_update --
+// rows: 0..2 first fused step, 3..5 last momentum step, 8 S_old (FP32),
+//       9 S_new, 10 K_new (fallback), 12..14 / 15..17 / 18 / 19: reversibility run
+static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exponent,
+                           std::initializer_list<int> protect, int rbase, int u_start,
+                           long double *K_old, long double *S_old, long double *K_new,
+                           long double *S_new, int *final_buf) {
+  const int nbf = su2k::fused_blocks(h->geo);
+  const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
+  const int pflags = h->aniso ? su2k::F_ANISO : 0;
+  h->cur = u_start;
+  if (h->prec == 1) CK(plaq_launch(h, u_start, -1, w_t, w_s, 0, pflags, rbase + 6));
+  IntegrateOut io;
+  CK(run_integrator(h, kind, nsteps, tau, exponent, protect, true, rbase, &io));
+  int cur = io.final_buf;
+  // restoreSU at the end of integrate (integrator.hh:48,111) fused with the
+  // accept step's action (md_update.hh:47-51): restored links go to a free buffer
+  int out;
+  if (h->group == 1) {
+    CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
+                    protect.size() > 1 ? *(protect.begin() + 1) : cur},
+                &out));
+    CK(plaq_launch(h, cur, out, w_t, w_s, 0, pflags | su2k::F_RESTORE, rbase + 7));
+    cur = out;
+  } else {
+    CK(plaq_launch(h, cur, -1, w_t, w_s, 0, pflags, rbase + 7));
+  }
+  if (!io.last_p_only) CK(sumsq_launch(h, h->P, rbase + 8));
+  if (io.first_accum) CK(reduce_row(h, rbase + 0, nbf));
+  if (io.first_accum && h->prec == 0) CK(reduce_row(h, rbase + 1, nbf));
+  if (io.last_p_only) CK(reduce_row(h, rbase + 5, nbf));
+  CK(fetch_sums(h));
+  *K_old = 0.5L * sum_of(h, rbase + 0);
+  const long double sg = -(long double)h->beta / (long double)Nc(h);
+  *S_old = (h->prec == 0) ? sg * (sum_of(h, rbase + 1) / 4.0L) : sg * sum_of(h, rbase + 6);
+  *K_new = 0.5L * (io.last_p_only ? sum_of(h, rbase + 5) : sum_of(h, rbase + 8));
+  *S_new = sg * sum_of(h, rbase + 7);
+  *final_buf = cur;
+  return SU2GPU_OK;
+}
+
+int su2gpu_md_trajectory(su2gpu_lattice *h, const double *momenta, int integrator, size_t nsteps,
+                         double tau, size_t exponent, int revtest, su2gpu_md_result *res) {
+  CK(check(h));
+  if (!res) return fail(SU2GPU_EINVAL, "null result");
+  if (h->pending) return fail(SU2GPU_ESTATE, "previous trajectory not finished");
+  if (momenta) CK(set_mom_host(h, h->P, momenta));
+  const int u0 = h->cur;
+  long double Ko, So, Kn, Sn;
+  int unew;
+  CK(trajectory_sums(h, integrator, nsteps, tau, exponent, {u0}, 0, u0, &Ko, &So, &Kn, &Sn, &unew));
+  std::memset(res, 0, sizeof(*res));
+  res->kin_old = (double)Ko;
+  res->gauge_old = (double)So;
+  res->kin_new = (double)Kn;
+  res->gauge_new = (double)Sn;
+  res->deltaH = (double)((Kn - Ko) + (Sn - So));
+  res->deltadeltaH = NAN;
+  res->uniform = -1.0;
+  res->accept = 1;
+  if (revtest) {
+    // md_update.hh:62-72: flip P, integrate back from U_new, H - H_old
+    CK(su2gpu_flip_momenta(h));
+    long double Kr_o, Sr_o, Kr, Sr;
+    int urev;
+    CK(trajectory_sums(h, integrator, nsteps, tau, exponent, {u0, unew}, 12, unew, &Kr_o, &Sr_o,
+                       &Kr, &Sr, &urev));
+    res->deltadeltaH = (double)((Kr - Ko) + (Sr - So));
+    res->revtest = 1;
+  }
+  h->cur = u0;
+  h->pending = true;
+  h->pend_old = u0;
+  h->pend_new = unew;
+  return SU2GPU_OK;
+}
+
+int su2gpu_md_finish(su2gpu_lattice *h, int accept) {
+  CK(check(h));
+  if (!h->pending) return fail(SU2GPU_ESTATE, "no trajectory pending");
+  h->cur = accept ? h->pend_new : h->pend_old;
+  h->pending = false;
+  return SU2GPU_OK;
+}
+
+int su2gpu_md_update(su2gpu_lattice *h, uint64_t seed, int integrator, size_t nsteps, double tau,
+                     size_t exponent, int revtest, int momenta_source, su2gpu_md_result *res) {
+  CK(check(h));
+  if (!res) return fail(SU2GPU_EINVAL, "null result");
+  std::mt19937 engine(seed);
+  if (momenta_source == SU2GPU_MOMENTA_HOST_MT19937) {
+    CK(ensure_staging(h, (size_t)h->nlinks * h->nm));
+    host_normals(h, engine);
+    CK(set_mom_host(h, h->P, h->hstage));
+  } else if (momenta_source == SU2GPU_MOMENTA_DEVICE_PHILOX) {
+    {
+      Prof pr(h, "momenta");
+      su2k::launch_momenta_philox(h->prec, h->P, h->V, h->nd, h->nm, seed, 0, h->stream);
+    }
+    CK(last_launch("momenta"));
+  } else {
+    return fail(SU2GPU_EINVAL, "bad momenta source");
+  }
+  CK(su2gpu_md_trajectory(h, nullptr, integrator, nsteps, tau, exponent, revtest, res));
+  int accept = 1;
+  if (res->deltaH > 0) {
+    double u;
+    if (momenta_source == SU2GPU_MOMENTA_HOST_MT19937) {
+      std::uniform_real_distribution<double> uniform(0., 1.);
+      u = uniform(engine);
+    } else {
+      u = philox_accept_uniform(seed, 0);
+    }
+    res->uniform = u;
+    if (u > std::exp(-res->deltaH)) accept = 0;
+  }
+  res->accept = accept;
+  return su2gpu_md_finish(h, accept);
+}
+
+int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, double *Q) {
+  CK(check(h));
+  if (!E || !Q) return fail(SU2GPU_EINVAL, "null pointer");
+  const int nbf = su2k::fused_blocks(h->geo);
+  {
+    Prof pr(h, "energy_density");
+    su2k::launch_energy_density(h->group, h->prec, h->U[h->cur], h->geo, clover ? 1 : 0,
+                                ss ? 1 : 0, row(h, 20), h->row_stride, h->stream);
+  }
+  CK(last_launch("energy_density"));
+  CK(reduce_row(h, 20, nbf));
+  CK(reduce_row(h, 21, nbf));
+  CK(fetch_sums(h));
+  long double e = sum_of(h, 20), q = sum_of(h, 21);
+  // flat-energy_density.hh:152-163
+  double res = -(double)e / (double)h->V;
+  if (clover) res /= 16.;
+  double qq = -4. * (double)q / (32.0 * M_PI * M_PI);
+  if (clover) qq /= 16.;
+  if (h->nd == 3) qq = 0.0;
+  *E = res;
+  *Q = qq;
+  return SU2GPU_OK;
+}
+
+int su2gpu_runge_kutta(su2gpu_lattice *h, double eps) {
+  CK(check(h));
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
+  CK(ensure_deriv(h));
+  // flat-gradient_flow.hh:17-50 with Z in the deriv field
+  const double zfac[3] = {(-17.0) / (36.0), (8.0) / (9.0), (-3.0) / (4.0)};
+  const double expfac[3] = {-36.0 / 4. / 17.0, 1., -1.};
+  for (int f = 0; f < 3; ++f) {
+    su2k::FusedCoef c = base_coef(h);
+    if (h->prec == 1 && h->group == 1) c.flags |= su2k::F_RESTORE;
+    const double fac = 2. * Nc(h) * zfac[f] / h->beta;
+    c.coef = fac * h->beta / Nc(h);
+    c.ep = -1.0;
+    c.eu = -eps * expfac[f];
+    if (f == 0) c.flags |= su2k::F_PZERO;
+    int out;
+    CK(free_buf(h, {h->cur}, &out));
+    CK(fused(h, h->cur, out, h->deriv, c, -1));
+    h->cur = out;
+  }
+  return SU2GPU_OK;
+}
+
+int su2gpu_gradient_flow(su2gpu_lattice *h, double tmax, double eps, double tstart, double *rows,
+                         size_t max_rows, size_t *nrows) {
+  CK(check(h));
+  if (!rows || !nrows) return fail(SU2GPU_EINVAL, "null pointer");
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
+  // the flow runs on a copy Vt of the links (flat-gradient_flow.hh:116-118)
+  const int orig = h->cur;
+  int vt;
+  CK(free_buf(h, {orig}, &vt));
+  HC(hipMemcpyAsync(h->U[vt], h->U[orig], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+  // ping-pong on the two buffers other than orig
+  const int d = h->nd;
+  const double ndf = d * (d - 1) / 2.0, ndf_ss = (d - 1) * (d - 2) / 2.0;
+  const double den = (double)h->V * Nc(h);
+  double t[3], P[3] = {0, 0, 0}, E[3] = {0, 0, 0}, Q[3] = {0, 0, 0}, Pss[3] = {0, 0, 0},
+               Ess[3] = {0, 0, 0};
+  for (int i = 0; i < 3; ++i) t[i] = tstart;
+  int rc = SU2GPU_OK;
+  auto measure = [&](int k, bool withQ) -> int {
+    double g, gss, e, q, ess, qss;
+    CK(su2gpu_gauge_energy(h, 0, &g));
+    CK(su2gpu_gauge_energy(h, 1, &gss));
+    CK(su2gpu_energy_density(h, 1, 0, &e, &q));
+    CK(su2gpu_energy_density(h, 1, 1, &ess, &qss));
+    P[k] = g / den;
+    Pss[k] = gss / den;
+    E[k] = e;
+    Ess[k] = ess;
+    if (withQ) Q[k] = q;
+    return SU2GPU_OK;
+  };
+  h->cur = vt;
+  size_t n = 0;
+  rc = measure(2, false);
+  while (rc == SU2GPU_OK && t[1] < tmax - eps) {
+    t[0] = t[2]; P[0] = P[2]; E[0] = E[2]; Q[0] = Q[2]; Pss[0] = Pss[2]; Ess[0] = Ess[2];
+    for (int k = 1; k < 3 && rc == SU2GPU_OK; ++k) {
+      t[k] = t[k - 1] + eps;
+      // keep orig untouched: runge_kutta picks free buffers other than cur; make
+      // sure orig is never chosen by temporarily marking it as pending-protected
+      rc = [&]() -> int {
+        CK(ensure_deriv(h));
+        const double zfac[3] = {(-17.0) / (36.0), (8.0) / (9.0), (-3.0) / (4.0)};
+        const double expfac[3] = {-36.0 / 4. / 17.0, 1., -1.};
+        for (int f = 0; f < 3; ++f) {
+          su2k::FusedCoef c = base_coef(h);
+          if (h->prec == 1 && h->group == 1) c.flags |= su2k::F_RESTORE;
+          const double fac = 2. * Nc(h) * zfac[f] / h->beta;
+          c.coef = fac * h->beta / Nc(h);
+          c.ep = -1.0;
+          c.eu = -eps * expfac[f];
+          if (f == 0) c.flags |= su2k::F_PZERO;
+          int out;
+          CK(free_buf(h, {h->cur, orig}, &out));
+          CK(fused(h, h->cur, out, h->deriv, c, -1));
+          h->cur = out;
+        }
+        return SU2GPU_OK;
+      }();
+      if (rc == SU2GPU_OK) rc = measure(k, true);
+    }
+    if (rc != SU2GPU_OK) break;
+    const double Ep = ndf - P[1], Ep_ss = ndf_ss - Pss[1];
+    const double xi2 = ((Ep - Ep_ss) / (d - 1)) / (Ep_ss / ndf_ss);
+    if (n < max_rows) {
+      double *r = rows + 9 * n;
+      r[0] = t[1]; r[1] = std::sqrt(xi2); r[2] = P[1]; r[3] = Pss[1]; r[4] = Ep; r[5] = Ep_ss;
+      r[6] = E[1]; r[7] = Ess[1]; r[8] = Q[1];
+    }
+    ++n;
+  }
+  h->cur = orig;
+  *nrows = n < max_rows ? n : max_rows;
+  return rc;
+}
+
+int su2gpu_save(su2gpu_lattice *h, const char *path) {
+  CK(check(h));
+  if (!path) return fail(SU2GPU_EINVAL, "null path");
+  const size_t rec = (size_t)(h->ncomp + 1);
+  std::vector<double> buf((size_t)h->nlinks * rec);
+  CK(su2gpu_get_links(h, buf.data(), SU2GPU_LAYOUT_OBJECT));
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(SU2GPU_EIO, "cannot open %s for writing", path);
+  const size_t w = std::fwrite(buf.data(), sizeof(double), buf.size(), f);
+  std::fclose(f);
+  if (w != buf.size()) return fail(SU2GPU_EIO, "short write to %s", path);
+  return SU2GPU_OK;
+}
+
+int su2gpu_load(su2gpu_lattice *h, const char *path) {
+  CK(check(h));
+  if (!path) return fail(SU2GPU_EINVAL, "null path");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(SU2GPU_EIO, "could not read file from %s", path);
+  const size_t rec = (size_t)(h->ncomp + 1);
+  std::vector<double> buf((size_t)h->nlinks * rec);
+  const size_t r = std::fread(buf.data(), sizeof(double), buf.size(), f);
+  std::fclose(f);
+  if (r != buf.size()) return fail(SU2GPU_EIO, "%s: expected %zu bytes, read %zu", path,
+                                   buf.size() * sizeof(double), r * sizeof(double));
+  return su2gpu_set_links(h, buf.data(), SU2GPU_LAYOUT_OBJECT);
+}
+
+int su2gpu_synchronize(su2gpu_lattice *h) {
+  CK(check(h));
+  HC(hipStreamSynchronize(h->stream));
+  return SU2GPU_OK;
+}
+
+int su2gpu_profile_enable(su2gpu_lattice *h, int on) {
+  CK(check(h));
+  CK(collect_prof(h));
+  h->prof = on != 0;
+  return SU2GPU_OK;
+}
+int su2gpu_profile_reset(su2gpu_lattice *h) {
+  CK(check(h));
+  CK(collect_prof(h));
+  h->prof_acc.clear();
+  return SU2GPU_OK;
+}
+int su2gpu_profile_read(su2gpu_lattice *h, const char *name, double *ms, long *count) {
+  CK(check(h));
+  if (!name || !ms || !count) return fail(SU2GPU_EINVAL, "null pointer");
+  CK(collect_prof(h));
+  auto it = h->prof_acc.find(name);
+  *ms = it == h->prof_acc.end() ? 0.0 : it->second.first;
+  *count = it == h->prof_acc.end() ? 0 : it->second.second;
+  return SU2GPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,581 @@
+// su2_kernels.hip -- hand-written gfx950 kernels for the pure-gauge HMC
+// molecular-dynamics step (staples, Wilson force, momentum/link updates,
+// plaquette / kinetic sums, clover energy density, layout conversion, device
+// momenta).  Every kernel is HBM-bound: no MFMA (2x2 complex products are not a
+// dense contraction).
+//
+// Field layout in HBM (one lattice):
+//   U : nd planes of V links, plane mu holds link (s, mu) at U[mu*V + s];
+//       SU(2) link = quat<T> (16 B FP32 / 32 B FP64, one or two 128-bit loads),
+//       U(1) link = T phase.
+//   P : nd*NMOM planes of V reals, component c of link (s,mu) at P[(mu*NMOM+c)*V+s].
+// Sites s follow the reference's lexicographic order (gaugeconfig.hh:211-222).
+#include "su2_device.hpp"
+#include "su2_launch.hpp"
+
+using namespace su2dev;
+
+namespace su2k {
+
+constexpr int NT = 256;  // threads per workgroup (4 wave64)
+
+// ----------------------------------------------------------------------------
+// Fused force + momentum update (+ link update).  One thread per site, all nd
+// link directions.  For every link (s, mu):
+//   K    = sum_{nu != mu} w_{mu nu} [upper + lower staple]   (get_staples.hh:26-80)
+//   F    = coef * get_deriv(U_mu(s) K)        (flat-gaugemonomial.hh:105-131)
+//   P   <- P - ep * F                          (update_momenta.hh:27-30)
+//   U'   = exp(eu * P) U_mu(s)  -> Uout        (update_gauge.hh:11-14)
+// Uin is read only; the link update goes to a different buffer (ping-pong), so
+// one launch is one "P then U" pair of the integrator with no grid barrier.
+// Optional block partial sums: |P_in|^2, ReTr(U K) and |P_out|^2 in FP64.
+// ----------------------------------------------------------------------------
+template <class G, int ND, bool WRITE_U>
+__global__ __launch_bounds__(NT) void k_md_fused(const typename G::link *__restrict__ Uin,
+                                                 typename G::link *__restrict__ Uout,
+                                                 typename G::real *__restrict__ P, Geo geo,
+                                                 FusedCoef c, double *__restrict__ partials,
+                                                 int pstride) {
+  using T = typename G::real;
+  using link = typename G::link;
+  using acc = typename G::acc;
+  constexpr int NM = G::NMOM;
+  __shared__ double lds[NT / 64];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NT + threadIdx.x;
+  const int V = geo.V;
+  double kin_old = 0.0, act = 0.0, kin_new = 0.0;
+  if (s < V) {
+    int up[ND], dn[ND];
+    neighbour_offsets<ND>(geo, s, up, dn);
+    link own[ND];
+#pragma unroll
+    for (int nu = 0; nu < ND; ++nu) own[nu] = Uin[nu * V + s];
+    const T ep = T(c.ep), eu = T(c.eu), coef = T(c.coef);
+    const T rn = T(c.round_n);
+#pragma unroll
+    for (int mu = 0; mu < ND; ++mu) {
+      acc K = G::zero(), Kc = G::zero();
+#pragma unroll
+      for (int nu = 0; nu < ND; ++nu) {
+        if (nu == mu) continue;
+        const T w = T((mu == 0 || nu == 0) ? c.w_t : c.w_s);
+        acc t1 = G::upper(Uin[nu * V + s + up[mu]], Uin[mu * V + s + up[nu]], own[nu]);
+        acc t2 = G::lower(Uin[nu * V + s + up[mu] + dn[nu]], Uin[mu * V + s + dn[nu]],
+                          Uin[nu * V + s + dn[nu]]);
+        if (c.flags & F_ANISO) {
+          t1 = G::scale(w, t1);
+          t2 = G::scale(w, t2);
+        }
+        if (c.flags & F_COMPENSATED) {
+          G::add_comp(K, Kc, t1);
+          G::add_comp(K, Kc, t2);
+        } else {
+          K = G::add(G::add(K, t1), t2);
+        }
+      }
+      if (c.flags & F_COMPENSATED) K = G::add(K, Kc);
+      T d[NM];
+      T retr;
+      G::project(own[mu], K, d, retr);
+      act += double(retr);
+      T p[NM];
+#pragma unroll
+      for (int k = 0; k < NM; ++k) {
+        p[k] = (c.flags & F_PZERO) ? T(0) : P[(mu * NM + k) * V + s];
+        kin_old += double(p[k]) * double(p[k]);
+        T f = coef * d[k];
+        if (c.flags & F_ROUND) f = ::round(f * rn) / rn;
+        p[k] -= ep * f;
+        kin_new += double(p[k]) * double(p[k]);
+        P[(mu * NM + k) * V + s] = p[k];
+      }
+      if (WRITE_U) {
+        link u = own[mu];
+        if (c.flags & F_ROUND) u = G::round(u, rn);
+        link un = G::expmul(p, eu, u);
+        if (c.flags & F_RESTORE) un = G::restore(un);
+        Uout[mu * V + s] = un;
+      }
+    }
+  }
+  if (c.flags & F_ACCUM) {
+    double r0 = block_sum<NT>(kin_old, lds);
+    double r1 = block_sum<NT>(act, lds);
+    double r2 = block_sum<NT>(kin_new, lds);
+    if (threadIdx.x == 0) {
+      partials[0 * pstride + lb] = r0;
+      partials[1 * pstride + lb] = r1;
+      partials[2 * pstride + lb] = r2;
+    }
+  }
+}
+
+// U <- exp(eps P) U in place, per link (update_gauge.hh:8-16; round variant :18-27)
+template <class G>
+__global__ __launch_bounds__(NT) void k_update_gauge(typename G::link *__restrict__ U,
+                                                     const typename G::real *__restrict__ P,
+                                                     int V, int nd, double eps, double round_n,
+                                                     int flags) {
+  using T = typename G::real;
+  constexpr int NM = G::NMOM;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= V * nd) return;
+  const int mu = i / V, s = i - mu * V;
+  T p[NM];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) p[k] = P[(mu * NM + k) * V + s];
+  typename G::link u = U[i];
+  if (flags & F_ROUND) u = G::round(u, T(round_n));
+  u = G::expmul(p, T(eps), u);
+  if (flags & F_RESTORE) u = G::restore(u);
+  U[i] = u;
+}
+
+// restoreSU (gaugeconfig.hh:115-120) in place
+template <class G>
+__global__ __launch_bounds__(NT) void k_restore(typename G::link *__restrict__ U, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) U[i] = G::restore(U[i]);
+}
+
+__device__ __forceinline__ quat<double> to_d(const quat<float> &q) {
+  return quat<double>{double(q.ar), double(q.ai), double(q.br), double(q.bi)};
+}
+__device__ __forceinline__ quat<double> to_d(const quat<double> &q) { return q; }
+__device__ __forceinline__ double to_d(float x) { return double(x); }
+__device__ __forceinline__ double to_d(double x) { return x; }
+
+// Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) computed in
+// FP64 whatever the storage precision.  With F_RESTORE each link is restored on
+// load and the restored links of the site are written to Uout (the end of an
+// integrate() followed by the accept step's action, md_update.hh:47-51).
+template <class GS, class GD, int ND>
+__global__ __launch_bounds__(NT) void k_plaq_sum(const typename GS::link *__restrict__ Uin,
+                                                 typename GS::link *__restrict__ Uout, Geo geo,
+                                                 double w_t, double w_s, int mu_start, int flags,
+                                                 double *__restrict__ partials) {
+  using linkd = typename GD::link;
+  __shared__ double lds[NT / 64];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NT + threadIdx.x;
+  const int V = geo.V;
+  double sum = 0.0;
+  if (s < V) {
+    int up[ND], dn[ND];
+    neighbour_offsets<ND>(geo, s, up, dn);
+    const bool rs = (flags & F_RESTORE) != 0;
+    linkd own[ND];
+#pragma unroll
+    for (int nu = 0; nu < ND; ++nu) {
+      typename GS::link u = Uin[nu * V + s];
+      if (rs) {
+        u = GS::restore(u);
+        Uout[nu * V + s] = u;
+      }
+      own[nu] = to_d(u);
+    }
+#pragma unroll
+    for (int mu = 0; mu < ND - 1; ++mu) {
+      if (mu < mu_start) continue;
+#pragma unroll
+      for (int nu = mu + 1; nu < ND; ++nu) {
+        typename GS::link a = Uin[nu * V + s + up[mu]], b = Uin[mu * V + s + up[nu]];
+        if (rs) {
+          a = GS::restore(a);
+          b = GS::restore(b);
+        }
+        const double v = GD::plaq(own[mu], to_d(a), to_d(b), own[nu]);
+        const double eta = (mu == 0) ? w_t : w_s;  // mu < nu: temporal iff mu == 0
+        sum += (flags & F_ANISO) ? eta * v : v;
+      }
+    }
+  }
+  const double r = block_sum<NT>(sum, lds);
+  if (threadIdx.x == 0) partials[lb] = r;
+}
+
+// 1/2 sum p^2 partials (kineticmonomial, monomial.hh:33-47), FP64 accumulation
+template <typename T>
+__global__ __launch_bounds__(NT) void k_sumsq(const T *__restrict__ P, int n,
+                                              double *__restrict__ partials) {
+  __shared__ double lds[NT / 64];
+  double v = 0.0;
+  for (int i = blockIdx.x * NT + threadIdx.x; i < n; i += gridDim.x * NT) {
+    const double x = double(P[i]);
+    v += x * x;
+  }
+  const double r = block_sum<NT>(v, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+template <typename T> __global__ __launch_bounds__(NT) void k_scale(T *__restrict__ P, int n, T f) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) P[i] = f * P[i];
+}
+
+// Deterministic final reduction of nval rows of nb block partials:
+// TwoSum-compensated strided accumulation, then a fixed-order LDS tree.
+// out[2*k] + out[2*k+1] is the sum of row k.
+__global__ __launch_bounds__(NT) void k_reduce(const double *__restrict__ partials, int nb,
+                                               double *__restrict__ out) {
+  __shared__ double hs[NT], ls[NT];
+  const double *row = partials + (size_t)blockIdx.x * nb;
+  double hi = 0.0, lo = 0.0;
+  for (int i = threadIdx.x; i < nb; i += NT) {
+    double e;
+    two_sum(hi, row[i], hi, e);
+    lo += e;
+  }
+  hs[threadIdx.x] = hi;
+  ls[threadIdx.x] = lo;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      double e, h;
+      two_sum(hs[threadIdx.x], hs[threadIdx.x + w], h, e);
+      hs[threadIdx.x] = h;
+      ls[threadIdx.x] = ls[threadIdx.x] + ls[threadIdx.x + w] + e;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double h, e;
+    two_sum(hs[0], ls[0], h, e);
+    out[2 * blockIdx.x] = h;
+    out[2 * blockIdx.x + 1] = e;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// clover energy density and topological charge (flat-energy_density.hh:57-164)
+// per site: G_{mu nu} = traceless-antihermitian part of the 4-leaf clover.
+// partials rows: 0 = sum ReTr(G G), 1 = sum eps ReTr(G G) (d=4) / -Im tr G01 (d=2)
+// ---------------------------------------------------------------------------
+template <class GS, int ND>
+__global__ __launch_bounds__(NT) void k_energy_density(const typename GS::link *__restrict__ Uin,
+                                                       Geo geo, int clover, int mu_start,
+                                                       double *__restrict__ partials, int pstride) {
+  __shared__ double lds[NT / 64];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NT + threadIdx.x;
+  const int V = geo.V;
+  double e_sum = 0.0, q_sum = 0.0;
+  if (s < V) {
+    int up[ND], dn[ND];
+    neighbour_offsets<ND>(geo, s, up, dn);
+    auto L = [&](int site, int d) { return to_d(Uin[d * V + site]); };
+    if constexpr (GS::NCOMP == 4) {
+      quat<double> Gm[ND][ND];
+#pragma unroll
+      for (int mu = 0; mu < ND - 1; ++mu)
+#pragma unroll
+        for (int nu = mu + 1; nu < ND; ++nu) {
+          if (mu < mu_start) { Gm[mu][nu] = quat<double>{0, 0, 0, 0}; continue; }
+          const int x1 = s + up[mu], x2 = s + up[nu];
+          quat<double> leaf = qmul(qmul(qmul(L(s, mu), L(x1, nu)), qdag(L(x2, mu))), qdag(L(s, nu)));
+          if (clover) {
+            const int a1 = s + dn[mu] + up[nu], a2 = s + dn[mu];
+            leaf = qadd(leaf, qmul(qmul(qmul(L(s, nu), qdag(L(a1, mu))), qdag(L(a2, nu))), L(a2, mu)));
+            const int b1 = s + dn[mu], b2 = s + dn[mu] + dn[nu], b3 = s + dn[nu];
+            leaf = qadd(leaf, qmul(qmul(qmul(qdag(L(b1, mu)), qdag(L(b2, nu))), L(b2, mu)), L(b3, nu)));
+            const int c1 = s + dn[nu], c2 = s + dn[nu] + up[mu];
+            leaf = qadd(leaf, qmul(qmul(qmul(qdag(L(c1, nu)), L(c1, mu)), L(c2, nu)), qdag(L(s, mu))));
+          }
+          // traceless_antiherm (su2.hh:83-85): (i Im a, b)
+          const quat<double> g = {0.0, leaf.ai, leaf.br, leaf.bi};
+          Gm[mu][nu] = g;
+          e_sum += 2.0 * qmul(g, g).ar;
+        }
+      if constexpr (ND == 4) {
+        // eps_{0123}-ordered sum over mu<nu, rho<sigma (tensors.hh:23-58):
+        // pairs (01,23), (02,13), (03,12) and their swaps, signs + - + + - +
+        const double t0123 = 2.0 * qmul(Gm[0][1], Gm[2][3]).ar;
+        const double t0213 = 2.0 * qmul(Gm[0][2], Gm[1][3]).ar;
+        const double t0312 = 2.0 * qmul(Gm[0][3], Gm[1][2]).ar;
+        const double t1203 = 2.0 * qmul(Gm[1][2], Gm[0][3]).ar;
+        const double t1302 = 2.0 * qmul(Gm[1][3], Gm[0][2]).ar;
+        const double t2301 = 2.0 * qmul(Gm[2][3], Gm[0][1]).ar;
+        q_sum = ((((t0123 - t0213) + t0312) + t1203) - t1302) + t2301;
+      }
+    } else {
+      double Gi[ND][ND];
+#pragma unroll
+      for (int mu = 0; mu < ND - 1; ++mu)
+#pragma unroll
+        for (int nu = mu + 1; nu < ND; ++nu) {
+          if (mu < mu_start) { Gi[mu][nu] = 0.0; continue; }
+          const int x1 = s + up[mu], x2 = s + up[nu];
+          double sn, cs, im = 0.0;
+          sincos(((L(s, mu) + L(x1, nu)) - L(x2, mu)) - L(s, nu), &sn, &cs);
+          im += sn;
+          if (clover) {
+            const int a1 = s + dn[mu] + up[nu], a2 = s + dn[mu];
+            im += sin(((L(s, nu) - L(a1, mu)) - L(a2, nu)) + L(a2, mu));
+            const int b1 = s + dn[mu], b2 = s + dn[mu] + dn[nu], b3 = s + dn[nu];
+            im += sin(((-L(b1, mu) - L(b2, nu)) + L(b2, mu)) + L(b3, nu));
+            const int c1 = s + dn[nu], c2 = s + dn[nu] + up[mu];
+            im += sin(((-L(c1, nu) + L(c1, mu)) + L(c2, nu)) - L(s, mu));
+          }
+          Gi[mu][nu] = im;
+          e_sum += -im * im;  // Re((i g)(i g))
+        }
+      if constexpr (ND == 4) {
+        const double t0123 = -Gi[0][1] * Gi[2][3], t0213 = -Gi[0][2] * Gi[1][3];
+        const double t0312 = -Gi[0][3] * Gi[1][2], t1203 = -Gi[1][2] * Gi[0][3];
+        const double t1302 = -Gi[1][3] * Gi[0][2], t2301 = -Gi[2][3] * Gi[0][1];
+        q_sum = ((((t0123 - t0213) + t0312) + t1203) - t1302) + t2301;
+      } else if constexpr (ND == 2) {
+        q_sum = -Gi[0][1];
+      }
+    }
+  }
+  const double r0 = block_sum<NT>(e_sum, lds);
+  const double r1 = block_sum<NT>(q_sum, lds);
+  if (threadIdx.x == 0) {
+    partials[lb] = r0;
+    partials[pstride + lb] = r1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion between the reference AoS order (site-major, mu fastest,
+// gaugeconfig.hh:211-222) held as doubles with a per-link stride, and the
+// direction-planar device layout.
+// ---------------------------------------------------------------------------
+template <typename T, int NC>
+__global__ __launch_bounds__(NT) void k_links_from_aos(const double *__restrict__ src, int stride,
+                                                       int offset, T *__restrict__ dst, int V,
+                                                       int nd) {
+  const int i = blockIdx.x * NT + threadIdx.x;  // AoS link index s*nd + mu
+  if (i >= V * nd) return;
+  const int s = i / nd, mu = i - s * nd;
+  const double *p = src + (size_t)i * stride + offset;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) dst[((size_t)mu * V + s) * NC + k] = T(p[k]);
+}
+template <typename T, int NC>
+__global__ __launch_bounds__(NT) void k_links_to_aos(const T *__restrict__ src, double *__restrict__ dst,
+                                                     int stride, int offset, int V, int nd) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= V * nd) return;
+  const int s = i / nd, mu = i - s * nd;
+  double *p = dst + (size_t)i * stride + offset;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) p[k] = double(src[((size_t)mu * V + s) * NC + k]);
+}
+// momenta: AoS (s, mu, c) <-> planar ((mu*NM + c)*V + s)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_mom_from_aos(const double *__restrict__ src, T *__restrict__ dst,
+                                                     int V, int nd, int nm) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= V * nd) return;
+  const int s = i / nd, mu = i - s * nd;
+  for (int k = 0; k < nm; ++k) dst[((size_t)mu * nm + k) * V + s] = T(src[(size_t)i * nm + k]);
+}
+template <typename T>
+__global__ __launch_bounds__(NT) void k_mom_to_aos(const T *__restrict__ src, double *__restrict__ dst,
+                                                   int V, int nd, int nm) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= V * nd) return;
+  const int s = i / nd, mu = i - s * nd;
+  for (int k = 0; k < nm; ++k) dst[(size_t)i * nm + k] = double(src[((size_t)mu * nm + k) * V + s]);
+}
+
+// Gaussian momenta from the counter-based Philox4x32-10 stream (device
+// throughput mode): link (s, mu) draws from counter (s*nd+mu, 2j, traj_lo,
+// traj_hi) with key = seed; Box-Muller on 53-bit uniforms.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_momenta_philox(T *__restrict__ P, int V, int nd, int nm,
+                                                       uint64_t seed, uint64_t traj) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= V * nd) return;
+  const int s = i / nd, mu = i - s * nd;
+  double z[4];
+  philox_normals4((uint32_t)i, (uint32_t)(traj & 0xffffffffu), (uint32_t)(traj >> 32), seed, z);
+  for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * V + s] = T(z[k]);
+}
+
+// ------------------------------------------------------------- launchers ----
+static inline int nblk(long n) { return (int)((n + NT - 1) / NT); }
+
+template <class G, int ND>
+static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
+  using link = typename G::link;
+  using T = typename G::real;
+  dim3 grid(nblk(L.geo.V)), block(NT);
+  if (L.Uout)
+    hipLaunchKernelGGL((k_md_fused<G, ND, true>), grid, block, 0, st, (const link *)L.Uin,
+                       (link *)L.Uout, (T *)L.P, L.geo, L.c, L.partials, L.pstride);
+  else
+    hipLaunchKernelGGL((k_md_fused<G, ND, false>), grid, block, 0, st, (const link *)L.Uin,
+                       (link *)nullptr, (T *)L.P, L.geo, L.c, L.partials, L.pstride);
+}
+
+template <class G> static void launch_fused_g(const FusedLaunch &L, hipStream_t st) {
+  switch (L.geo.nd) {
+    case 2: launch_fused_nd<G, 2>(L, st); break;
+    case 3: launch_fused_nd<G, 3>(L, st); break;
+    default: launch_fused_nd<G, 4>(L, st); break;
+  }
+}
+
+void launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st) {
+  if (group == 1) {
+    if (prec == 0) launch_fused_g<GSU2<double>>(L, st);
+    else launch_fused_g<GSU2<float>>(L, st);
+  } else {
+    if (prec == 0) launch_fused_g<GU1<double>>(L, st);
+    else launch_fused_g<GU1<float>>(L, st);
+  }
+}
+
+int fused_blocks(const Geo &g) { return nblk(g.V); }
+
+template <class G>
+static void launch_update_gauge_g(void *U, const void *P, int V, int nd, double eps,
+                                  double round_n, int flags, hipStream_t st) {
+  hipLaunchKernelGGL((k_update_gauge<G>), dim3(nblk((long)V * nd)), dim3(NT), 0, st,
+                     (typename G::link *)U, (const typename G::real *)P, V, nd, eps, round_n, flags);
+}
+void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int nd, double eps,
+                         double round_n, int flags, hipStream_t st) {
+  if (group == 1) {
+    if (prec == 0) launch_update_gauge_g<GSU2<double>>(U, P, V, nd, eps, round_n, flags, st);
+    else launch_update_gauge_g<GSU2<float>>(U, P, V, nd, eps, round_n, flags, st);
+  } else {
+    if (prec == 0) launch_update_gauge_g<GU1<double>>(U, P, V, nd, eps, round_n, flags, st);
+    else launch_update_gauge_g<GU1<float>>(U, P, V, nd, eps, round_n, flags, st);
+  }
+}
+
+void launch_restore(int group, int prec, void *U, int n, hipStream_t st) {
+  if (group != 1) return;
+  if (prec == 0)
+    hipLaunchKernelGGL((k_restore<GSU2<double>>), dim3(nblk(n)), dim3(NT), 0, st, (quat<double> *)U, n);
+  else
+    hipLaunchKernelGGL((k_restore<GSU2<float>>), dim3(nblk(n)), dim3(NT), 0, st, (quat<float> *)U, n);
+}
+
+template <class GS, class GD>
+static void launch_plaq_g(const void *Uin, void *Uout, const Geo &g, double w_t, double w_s,
+                          int mu_start, int flags, double *partials, hipStream_t st) {
+  using link = typename GS::link;
+  dim3 grid(nblk(g.V));
+  switch (g.nd) {
+    case 2:
+      hipLaunchKernelGGL((k_plaq_sum<GS, GD, 2>), grid, dim3(NT), 0, st, (const link *)Uin,
+                         (link *)Uout, g, w_t, w_s, mu_start, flags, partials);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_plaq_sum<GS, GD, 3>), grid, dim3(NT), 0, st, (const link *)Uin,
+                         (link *)Uout, g, w_t, w_s, mu_start, flags, partials);
+      break;
+    default:
+      hipLaunchKernelGGL((k_plaq_sum<GS, GD, 4>), grid, dim3(NT), 0, st, (const link *)Uin,
+                         (link *)Uout, g, w_t, w_s, mu_start, flags, partials);
+      break;
+  }
+}
+void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
+                 double w_s, int mu_start, int flags, double *partials, hipStream_t st) {
+  if (group == 1) {
+    if (prec == 0) launch_plaq_g<GSU2<double>, GSU2<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
+    else launch_plaq_g<GSU2<float>, GSU2<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
+  } else {
+    if (prec == 0) launch_plaq_g<GU1<double>, GU1<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
+    else launch_plaq_g<GU1<float>, GU1<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
+  }
+}
+
+int sumsq_blocks(long n) {
+  int b = nblk(n);
+  return b < 2048 ? b : 2048;
+}
+void launch_sumsq(int prec, const void *P, long n, double *partials, hipStream_t st) {
+  const int b = sumsq_blocks(n);
+  if (prec == 0)
+    hipLaunchKernelGGL((k_sumsq<double>), dim3(b), dim3(NT), 0, st, (const double *)P, (int)n, partials);
+  else
+    hipLaunchKernelGGL((k_sumsq<float>), dim3(b), dim3(NT), 0, st, (const float *)P, (int)n, partials);
+}
+
+void launch_scale(int prec, void *P, long n, double f, hipStream_t st) {
+  if (prec == 0)
+    hipLaunchKernelGGL((k_scale<double>), dim3(nblk(n)), dim3(NT), 0, st, (double *)P, (int)n, f);
+  else
+    hipLaunchKernelGGL((k_scale<float>), dim3(nblk(n)), dim3(NT), 0, st, (float *)P, (int)n, (float)f);
+}
+
+void launch_reduce(const double *partials, int nb, int nrows, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce, dim3(nrows), dim3(NT), 0, st, partials, nb, out);
+}
+
+template <class GS>
+static void launch_ed_g(const void *U, const Geo &g, int clover, int mu_start, double *partials,
+                        int pstride, hipStream_t st) {
+  using link = typename GS::link;
+  dim3 grid(nblk(g.V));
+  switch (g.nd) {
+    case 2:
+      hipLaunchKernelGGL((k_energy_density<GS, 2>), grid, dim3(NT), 0, st, (const link *)U, g, clover, mu_start, partials, pstride);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_energy_density<GS, 3>), grid, dim3(NT), 0, st, (const link *)U, g, clover, mu_start, partials, pstride);
+      break;
+    default:
+      hipLaunchKernelGGL((k_energy_density<GS, 4>), grid, dim3(NT), 0, st, (const link *)U, g, clover, mu_start, partials, pstride);
+      break;
+  }
+}
+void launch_energy_density(int group, int prec, const void *U, const Geo &g, int clover,
+                           int mu_start, double *partials, int pstride, hipStream_t st) {
+  if (group == 1) {
+    if (prec == 0) launch_ed_g<GSU2<double>>(U, g, clover, mu_start, partials, pstride, st);
+    else launch_ed_g<GSU2<float>>(U, g, clover, mu_start, partials, pstride, st);
+  } else {
+    if (prec == 0) launch_ed_g<GU1<double>>(U, g, clover, mu_start, partials, pstride, st);
+    else launch_ed_g<GU1<float>>(U, g, clover, mu_start, partials, pstride, st);
+  }
+}
+
+void launch_links_from_aos(int group, int prec, const double *src, int stride, int offset, void *dst,
+                           int V, int nd, hipStream_t st) {
+  dim3 grid(nblk((long)V * nd));
+  if (group == 1) {
+    if (prec == 0) hipLaunchKernelGGL((k_links_from_aos<double, 4>), grid, dim3(NT), 0, st, src, stride, offset, (double *)dst, V, nd);
+    else hipLaunchKernelGGL((k_links_from_aos<float, 4>), grid, dim3(NT), 0, st, src, stride, offset, (float *)dst, V, nd);
+  } else {
+    if (prec == 0) hipLaunchKernelGGL((k_links_from_aos<double, 1>), grid, dim3(NT), 0, st, src, stride, offset, (double *)dst, V, nd);
+    else hipLaunchKernelGGL((k_links_from_aos<float, 1>), grid, dim3(NT), 0, st, src, stride, offset, (float *)dst, V, nd);
+  }
+}
+void launch_links_to_aos(int group, int prec, const void *src, double *dst, int stride, int offset,
+                         int V, int nd, hipStream_t st) {
+  dim3 grid(nblk((long)V * nd));
+  if (group == 1) {
+    if (prec == 0) hipLaunchKernelGGL((k_links_to_aos<double, 4>), grid, dim3(NT), 0, st, (const double *)src, dst, stride, offset, V, nd);
+    else hipLaunchKernelGGL((k_links_to_aos<float, 4>), grid, dim3(NT), 0, st, (const float *)src, dst, stride, offset, V, nd);
+  } else {
+    if (prec == 0) hipLaunchKernelGGL((k_links_to_aos<double, 1>), grid, dim3(NT), 0, st, (const double *)src, dst, stride, offset, V, nd);
+    else hipLaunchKernelGGL((k_links_to_aos<float, 1>), grid, dim3(NT), 0, st, (const float *)src, dst, stride, offset, V, nd);
+  }
+}
+void launch_mom_from_aos(int prec, const double *src, void *dst, int V, int nd, int nm, hipStream_t st) {
+  dim3 grid(nblk((long)V * nd));
+  if (prec == 0) hipLaunchKernelGGL((k_mom_from_aos<double>), grid, dim3(NT), 0, st, src, (double *)dst, V, nd, nm);
+  else hipLaunchKernelGGL((k_mom_from_aos<float>), grid, dim3(NT), 0, st, src, (float *)dst, V, nd, nm);
+}
+void launch_mom_to_aos(int prec, const void *src, double *dst, int V, int nd, int nm, hipStream_t st) {
+  dim3 grid(nblk((long)V * nd));
+  if (prec == 0) hipLaunchKernelGGL((k_mom_to_aos<double>), grid, dim3(NT), 0, st, (const double *)src, dst, V, nd, nm);
+  else hipLaunchKernelGGL((k_mom_to_aos<float>), grid, dim3(NT), 0, st, (const float *)src, dst, V, nd, nm);
+}
+void launch_momenta_philox(int prec, void *P, int V, int nd, int nm, uint64_t seed, uint64_t traj,
+                           hipStream_t st) {
+  dim3 grid(nblk((long)V * nd));
+  if (prec == 0) hipLaunchKernelGGL((k_momenta_philox<double>), grid, dim3(NT), 0, st, (double *)P, V, nd, nm, seed, traj);
+  else hipLaunchKernelGGL((k_momenta_philox<float>), grid, dim3(NT), 0, st, (float *)P, V, nd, nm, seed, traj);
+}
+
+}  // namespace su2k

@@ -1,0 +1,117 @@
+// su2_launch.hpp -- launch interface between the C-ABI host code and the
+// gfx950 kernels, plus the counter-based Philox4x32-10 generator shared by host
+// and device (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+// 1, 2, 3", SC'11: 10 rounds, multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key
+// increments 0x9E3779B9 / 0xBB67AE85).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "su2_device.hpp"
+
+namespace su2k {
+
+using su2dev::Geo;
+
+// flags of the fused MD kernel and of the sum kernels
+enum : int {
+  F_PZERO = 1,         // momentum input treated as zero (gradient-flow stage 0)
+  F_ACCUM = 2,         // write block partials of |P_in|^2, ReTr(U K), |P_out|^2
+  F_RESTORE = 4,       // restoreSU on the updated link / on load
+  F_COMPENSATED = 8,   // TwoSum-compensated staple accumulation
+  F_ROUND = 16,        // lp_* integrators: round force and link (update_*.hh round variants)
+  F_ANISO = 32,        // anisotropic staple / plaquette weights
+};
+
+struct FusedCoef {
+  double ep;       // P <- P - ep * F
+  double eu;       // U' = exp(eu P) U
+  double coef;     // fac * beta / Nc
+  double w_t;      // weight of staples/plaquettes touching direction 0 (1/xi)
+  double w_s;      // weight of purely spatial ones (xi)
+  double round_n;  // rounding multiplier 10^exponent (F_ROUND)
+  int flags;
+};
+
+struct FusedLaunch {
+  const void *Uin;
+  void *Uout;  // nullptr: momentum-only step
+  void *P;
+  Geo geo;
+  FusedCoef c;
+  double *partials;  // 3 rows of fused_blocks(geo) doubles (F_ACCUM) ...
+  int pstride;       // ... pstride doubles apart
+};
+
+void launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st);
+int fused_blocks(const Geo &g);
+void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int nd, double eps,
+                         double round_n, int flags, hipStream_t st);
+void launch_restore(int group, int prec, void *U, int n, hipStream_t st);
+void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
+                 double w_s, int mu_start, int flags, double *partials, hipStream_t st);
+int sumsq_blocks(long n);
+void launch_sumsq(int prec, const void *P, long n, double *partials, hipStream_t st);
+void launch_scale(int prec, void *P, long n, double f, hipStream_t st);
+void launch_reduce(const double *partials, int nb, int nrows, double *out, hipStream_t st);
+void launch_energy_density(int group, int prec, const void *U, const Geo &g, int clover,
+                           int mu_start, double *partials, int pstride, hipStream_t st);
+void launch_links_from_aos(int group, int prec, const double *src, int stride, int offset,
+                           void *dst, int V, int nd, hipStream_t st);
+void launch_links_to_aos(int group, int prec, const void *src, double *dst, int stride,
+                         int offset, int V, int nd, hipStream_t st);
+void launch_mom_from_aos(int prec, const double *src, void *dst, int V, int nd, int nm,
+                         hipStream_t st);
+void launch_mom_to_aos(int prec, const void *src, double *dst, int V, int nd, int nm,
+                       hipStream_t st);
+void launch_momenta_philox(int prec, void *P, int V, int nd, int nm, uint64_t seed,
+                           uint64_t traj, hipStream_t st);
+
+}  // namespace su2k
+
+// ----------------------------------------------------------------- Philox ---
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)M0 * c[0];
+    const uint64_t p1 = (uint64_t)M1 * c[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += W0;
+    k1 += W1;
+  }
+}
+
+// 53-bit uniform in (0, 1) from two 32-bit words
+__host__ __device__ inline double philox_u01(uint32_t hi, uint32_t lo) {
+  const uint64_t m = (((uint64_t)hi << 32) | lo) >> 11;
+  return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// four N(0,1) draws for counter (i, 0|1, t0, t1) and key = seed (Box-Muller)
+__host__ __device__ inline void philox_normals4(uint32_t i, uint32_t t0, uint32_t t1, uint64_t seed,
+                                                double z[4]) {
+  for (int h = 0; h < 2; ++h) {
+    uint32_t c[4] = {i, (uint32_t)h, t0, t1};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = philox_u01(c[0], c[1]), u2 = philox_u01(c[2], c[3]);
+    const double r = sqrt(-2.0 * log(u1));
+    const double th = 6.283185307179586 * u2;
+    z[2 * h] = r * cos(th);
+    z[2 * h + 1] = r * sin(th);
+  }
+}
+
+// the accept/reject uniform of trajectory traj in the device-momenta mode
+__host__ __device__ inline double philox_accept_uniform(uint64_t seed, uint64_t traj) {
+  uint32_t c[4] = {0xffffffffu, 0xffffffffu, (uint32_t)traj, (uint32_t)(traj >> 32)};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  return philox_u01(c[0], c[1]);
+}

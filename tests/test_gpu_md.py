@@ -1,0 +1,296 @@
+"""GPU parity tests of the MD path (run on an MI355X: pytest -m gpu).
+
+Every test calls the HIP library through the C ABI (su2_amd binding) and compares
+with the CPU oracle (oracle/su2_oracle.c, itself pinned to the reference by
+tests/test_oracle.py) on identical seeded inputs.
+
+Tolerances (FP64 form): per-link and per-momentum 1e-12 for one force / update
+sweep, 1e-10 after a full trajectory; dH to 1e-9 relative (both sides use
+compensated sums); plaquette and energy density to 1e-12 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 13526463
+
+
+@pytest.fixture(scope="module")
+def S():
+    import su2_amd
+    su2_amd.load_library()
+    assert su2_amd.device_count() >= 1
+    return su2_amd
+
+
+def _start(oracle, grp, dims, seed=SEED, delta=1.0):
+    U = oracle.hotstart(grp, dims, seed, delta)
+    return oracle.random_gauge_trafo(grp, dims, U, 654321)
+
+
+GEOMS = [(1, (4, 4, 4, 4, 4)), (1, (3, 4, 5, 6, 4)), (1, (6, 1, 1, 5, 2)), (1, (4, 5, 1, 3, 3)),
+         (0, (4, 4, 4, 4, 4)), (0, (8, 1, 1, 6, 2)), (0, (4, 5, 1, 3, 3))]
+
+
+def test_start_configuration_bit_exact(S, oracle, golden):
+    with S.Lattice(S.SU2, (4, 4, 4, 4, 4), 2.3) as lat:
+        lat.hotstart(SEED)
+        lat.random_gauge_trafo(654321)
+        np.testing.assert_array_equal(lat.get_links(), golden["su2_4444_start"])
+    with S.Lattice(S.U1, (8, 1, 1, 8, 2), 2.0) as lat:
+        lat.hotstart(SEED)
+        lat.random_gauge_trafo(654321)
+        np.testing.assert_array_equal(lat.get_links(), golden["u1_88_start"])
+
+
+def test_initnormal_bit_exact(S, oracle):
+    dims = (4, 4, 4, 4, 4)
+    with S.Lattice(S.SU2, dims, 2.3) as lat:
+        lat.initnormal(SEED)
+        np.testing.assert_array_equal(lat.get_momenta(), oracle.initnormal(1, dims, SEED))
+
+
+@pytest.mark.parametrize("grp,dims", GEOMS)
+@pytest.mark.parametrize("xi", [1.0, 1.3])
+def test_derivative_parity(S, oracle, grp, dims, xi):
+    U = _start(oracle, grp, dims)
+    D0 = oracle.initnormal(grp, dims, 3)
+    want = oracle.derivative(grp, dims, U, 2.3, xi, fac=0.7, deriv=D0)
+    with S.Lattice(grp, dims, 2.3, xi=xi) as lat:
+        lat.set_links(U)
+        got = lat.derivative(0.7, D0)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(1.0, np.abs(want).max()))
+
+
+@pytest.mark.parametrize("grp,dims", GEOMS)
+def test_sums_parity(S, oracle, grp, dims):
+    U = _start(oracle, grp, dims)
+    P = oracle.initnormal(grp, dims, 4)
+    with S.Lattice(grp, dims, 2.3) as lat:
+        lat.set_links(U)
+        lat.set_momenta(P)
+        for sp in (False, True):
+            if dims[4] == 2 and sp:
+                continue
+            assert lat.gauge_energy(sp) == pytest.approx(
+                oracle.retr_sum_plaq(grp, dims, U, spatial_only=sp, compensated=True), rel=1e-13, abs=1e-12)
+        assert lat.gauge_action() == pytest.approx(oracle.action(grp, dims, U, 2.3, compensated=True), rel=1e-13)
+        assert lat.kinetic_energy() == pytest.approx(oracle.kinetic(grp, dims, P, compensated=True), rel=1e-14)
+        assert lat.retr_sum_plaquettes(1.4, True) == pytest.approx(
+            oracle.retr_sum_plaq(grp, dims, U, xi=1.4, compensated=True), rel=1e-13, abs=1e-12)
+
+
+@pytest.mark.parametrize("grp,dims", GEOMS)
+def test_update_momenta_and_gauge_parity(S, oracle, grp, dims):
+    U = _start(oracle, grp, dims)
+    P = oracle.initnormal(grp, dims, 4)
+    P1 = oracle.update_momenta(grp, dims, U, P, 2.3, 0.05)
+    U1_ = oracle.update_gauge(grp, dims, U, P1, 0.05)
+    with S.Lattice(grp, dims, 2.3) as lat:
+        lat.set_links(U)
+        lat.set_momenta(P)
+        lat.update_momenta(0.05)
+        np.testing.assert_allclose(lat.get_momenta(), P1, atol=1e-12)
+        lat.update_gauge(0.05)
+        np.testing.assert_allclose(lat.get_links(), U1_, atol=1e-13)
+
+
+@pytest.mark.parametrize("kind", range(7))
+@pytest.mark.parametrize("grp,dims,xi", [(1, (4, 4, 4, 4, 4), 1.0), (1, (4, 3, 2, 5, 4), 1.2),
+                                         (0, (8, 1, 1, 6, 2), 1.0), (1, (4, 5, 1, 3, 3), 1.0)])
+def test_integrators_parity(S, oracle, kind, grp, dims, xi):
+    U = _start(oracle, grp, dims)
+    P = oracle.initnormal(grp, dims, 6)
+    Uw, Pw = oracle.integrate(grp, dims, U, P, 2.3, kind, 5, xi=xi, exponent=9)
+    with S.Lattice(grp, dims, 2.3, xi=xi) as lat:
+        lat.set_links(U)
+        lat.set_momenta(P)
+        lat.integrate(kind, 5, exponent=9)
+        Ug, Pg = lat.get_links(), lat.get_momenta()
+    tol = 1e-6 if kind in (1, 3) else 1e-11  # lp_*: rounding at 1e-9 may flip a last digit
+    np.testing.assert_allclose(Ug, Uw, atol=tol)
+    np.testing.assert_allclose(Pg, Pw, atol=tol * 10)
+
+
+@pytest.mark.parametrize("kind,n,beta", [(0, 10, 2.3), (6, 20, 2.4), (2, 10, 2.4)])
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4, 4), (8, 8, 8, 8, 4)])
+def test_md_update_parity(S, oracle, kind, n, beta, dims):
+    U = _start(oracle, 1, dims)
+    Uw, info = oracle.md_update(1, dims, U, beta, kind, n, SEED + 3, revtest=True, compensated=True)
+    with S.Lattice(S.SU2, dims, beta) as lat:
+        lat.set_links(U)
+        r = lat.md_update(SEED + 3, kind, n, revtest=True)
+        Ug = lat.get_links()
+    assert bool(r["accept"]) == info["accept"]
+    assert r["deltaH"] == pytest.approx(info["dH"], rel=1e-9, abs=1e-11)
+    assert r["deltadeltaH"] == pytest.approx(info["ddH"], abs=1e-9)
+    assert r["kin_old"] == pytest.approx(info["K_old"], rel=1e-14)
+    assert r["gauge_old"] == pytest.approx(info["S_old"], rel=1e-13)
+    np.testing.assert_allclose(Ug, Uw, atol=1e-10)
+
+
+def test_md_update_u1(S, oracle):
+    dims = (16, 1, 1, 16, 2)
+    U = _start(oracle, 0, dims)
+    Uw, info = oracle.md_update(0, dims, U, 2.0, 0, 10, SEED, compensated=True)
+    with S.Lattice(S.U1, dims, 2.0) as lat:
+        lat.set_links(U)
+        r = lat.md_update(SEED, 0, 10)
+        np.testing.assert_allclose(lat.get_links(), Uw, atol=1e-10)
+    assert r["deltaH"] == pytest.approx(info["dH"], rel=1e-9, abs=1e-11)
+    assert bool(r["accept"]) == info["accept"]
+
+
+def test_split_trajectory_reject_restores(S, oracle):
+    dims = (4, 4, 4, 4, 4)
+    U = _start(oracle, 1, dims)
+    P = oracle.initnormal(1, dims, 9)
+    with S.Lattice(S.SU2, dims, 2.4) as lat:
+        lat.set_links(U)
+        r = lat.md_trajectory(P, 6, 5)
+        lat.md_finish(False)
+        np.testing.assert_array_equal(lat.get_links(), U)
+        r2 = lat.md_trajectory(P, 6, 5)
+        lat.md_finish(True)
+        assert r2["deltaH"] == r["deltaH"]
+        Uw, _ = oracle.integrate(1, dims, U, P, 2.4, 6, 5)
+        np.testing.assert_allclose(lat.get_links(), Uw, atol=1e-11)
+
+
+def test_hmc_program_8888(S, oracle, golden):
+    """The 8^4 beta=2.3 leapfrog(10), 20-trajectory run of the HMC program
+    (hmc.hpp:86-129) driven through the C ABI, against the reference's rows."""
+    dims = (8, 8, 8, 8, 4)
+    ref = golden["hmc_8888_b2.3_lf10_rows"]
+    with S.Lattice(S.SU2, dims, 2.3) as lat:
+        lat.hotstart(SEED, 1.0)
+        lat.random_gauge_trafo(654321)
+        norm = 2.0 / 4 / 3 / lat.V / 2.0
+        rate = 0.0
+        for i in range(20):
+            rev = i > 0 and i % 5 == 0
+            r = lat.md_update(SEED + i, 0, 10, revtest=rev)
+            plaq = lat.gauge_energy() * norm
+            E, Q = lat.energy_density()
+            rate += r["accept"]
+            assert r["accept"] == ref[i, 1]
+            assert plaq == pytest.approx(ref[i, 2], rel=1e-11)
+            assert r["deltaH"] == pytest.approx(ref[i, 3], abs=5e-9)
+            assert rate / (i + 1) == pytest.approx(ref[i, 4])
+            if rev:
+                assert r["deltadeltaH"] == pytest.approx(ref[i, 5], abs=1e-9)
+            assert Q == pytest.approx(ref[i, 6], rel=1e-9, abs=1e-11)
+
+
+@pytest.mark.parametrize("grp,dims", [(1, (4, 4, 4, 4, 4)), (1, (4, 5, 1, 3, 3)), (0, (4, 4, 4, 4, 4)),
+                                      (0, (8, 1, 1, 6, 2)), (1, (6, 1, 1, 5, 2))])
+def test_energy_density_parity(S, oracle, grp, dims):
+    U = _start(oracle, grp, dims)
+    with S.Lattice(grp, dims, 2.3) as lat:
+        lat.set_links(U)
+        for clover in (True, False):
+            for ss in (False, True):
+                e, q = lat.energy_density(clover, ss)
+                ew, qw = oracle.energy_density(grp, dims, U, clover, ss)
+                assert e == pytest.approx(ew, rel=1e-12, abs=1e-14)
+                assert q == pytest.approx(qw, rel=1e-10, abs=1e-13)
+
+
+def test_u1_charge_known_answer_on_gpu(S, oracle):
+    dims = (4, 4, 4, 4, 4)
+    U = np.zeros(4 ** 4 * 4)
+
+    def idx(t, x, y, z, mu):
+        return (((t * 4 + x) * 4 + y) * 4 + z) * 4 + mu
+    pi = math.pi
+    for (c, v) in [((1, 1, 1, 1, 1), pi / 2), ((1, 1, 1, 1, 2), pi / 2), ((0, 1, 1, 1, 0), pi),
+                   ((1, 0, 1, 1, 1), pi / 2), ((1, 1, 0, 0, 2), pi), ((1, 1, 1, 0, 3), pi / 2)]:
+        U[idx(*c)] = v
+    with S.Lattice(S.U1, dims, 1.0) as lat:
+        lat.set_links(U)
+        _, Q = lat.energy_density()
+    assert Q == pytest.approx(0.0126651, abs=5e-8)
+
+
+@pytest.mark.parametrize("grp", [1, 0])
+def test_runge_kutta_and_flow_parity(S, oracle, grp, golden):
+    dims = (4, 4, 4, 4, 4)
+    U = _start(oracle, grp, dims)
+    with S.Lattice(grp, dims, 2.4) as lat:
+        lat.set_links(U)
+        rows = lat.gradient_flow(0.1, 0.01)
+        np.testing.assert_array_equal(lat.get_links(), U)  # the flow works on a copy
+        lat.runge_kutta(0.01)
+        np.testing.assert_allclose(lat.get_links(), oracle.runge_kutta(grp, dims, U, 2.4, 0.01), atol=1e-13)
+    _, want = oracle.gradient_flow(grp, dims, U, 2.4, 0.1, 0.01)
+    assert rows.shape == want.shape
+    np.testing.assert_allclose(rows, want, rtol=1e-10, atol=1e-12)
+    if grp == 1:
+        np.testing.assert_allclose(rows, golden["su2_4444_flow_rows"], rtol=1e-10, atol=1e-12)
+
+
+def test_save_load_reference_format(S, oracle, tmp_path):
+    dims = (4, 4, 4, 4, 4)
+    U = _start(oracle, 1, dims)
+    p = str(tmp_path / "conf.0")
+    with S.Lattice(S.SU2, dims, 2.3) as lat:
+        lat.set_links(U)
+        lat.save(p)
+    raw = open(p, "rb").read()
+    assert len(raw) == 4 ** 4 * 4 * 40  # sizeof(_su2) = 40: size_t N_c, complex a, complex b
+    rec = np.frombuffer(raw, dtype=np.float64).reshape(-1, 5)
+    assert np.all(np.frombuffer(raw, dtype=np.uint64).reshape(-1, 5)[:, 0] == 2)
+    np.testing.assert_array_equal(rec[:, 1:], U)
+    with S.Lattice(S.SU2, dims, 2.3) as lat:
+        lat.load(p)
+        np.testing.assert_array_equal(lat.get_links(), U)
+        with pytest.raises(S.SU2GPUError):
+            lat.load(str(tmp_path / "missing"))
+
+
+def test_fp32_form_tracks_fp64(S, oracle):
+    dims = (8, 8, 8, 8, 4)
+    U = _start(oracle, 1, dims)
+    Uw, info = oracle.md_update(1, dims, U, 2.4, 6, 20, SEED, compensated=True)
+    with S.Lattice(S.SU2, dims, 2.4, precision=S.FP32) as lat:
+        lat.set_links(U)
+        r = lat.md_update(SEED, 6, 20, revtest=True)
+        Ug = lat.get_links()
+    assert abs(r["deltaH"] - info["dH"]) < 2e-3
+    assert abs(r["deltadeltaH"]) < 2e-3
+    assert np.abs(Ug - Uw).max() < 1e-4
+
+
+def test_device_momenta_statistics(S):
+    dims = (8, 8, 8, 8, 4)
+    with S.Lattice(S.SU2, dims, 2.4) as lat:
+        lat.hotstart(SEED)
+        lat.md_update(SEED, 6, 4, momenta=S.MOMENTA_DEVICE)
+        r = lat.md_update(SEED + 1, 6, 4, momenta=S.MOMENTA_DEVICE)
+        assert math.isfinite(r["deltaH"])
+        n = lat.nlinks * 3
+        K = r["kin_old"]
+        assert abs(2 * K / n - 1.0) < 5 * math.sqrt(2.0 / n)
+
+
+def test_gauge_invariance_and_reversibility_32(S, oracle):
+    """Size-independent properties at the benchmark size 32^4."""
+    dims = (32, 32, 32, 32, 4)
+    with S.Lattice(S.SU2, dims, 2.4) as lat:
+        lat.hotstart(SEED)
+        s0 = lat.gauge_action()
+        e0, _ = lat.energy_density()
+        lat.random_gauge_trafo(654321)
+        assert lat.gauge_action() == pytest.approx(s0, rel=1e-12)
+        assert lat.energy_density()[0] == pytest.approx(e0, rel=1e-11)
+        U = lat.get_links()
+        # one force sweep against the oracle at full size
+        D = lat.derivative(1.0)
+        want = oracle.derivative(1, dims, U, 2.4)
+        np.testing.assert_allclose(D, want, atol=1e-12 * np.abs(want).max())
+        r = lat.md_update(SEED, 6, 20, revtest=True)
+        assert abs(r["deltadeltaH"]) < 1e-7
+        assert abs(r["deltaH"]) < 5.0
