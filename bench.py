@@ -84,15 +84,16 @@ def run_form(S, prec, L, nsteps, steps, warmup, seed, beta):
     }
 
 
-def delta_dH(S, L, nsteps, beta, seed, ntraj):
-    """FP32 vs FP64 from identical start and momenta, and FP32 reversibility."""
+def delta_dH(S, L, nsteps, beta, seed, ntraj, ntherm=30):
+    """FP32 vs FP64 from identical (thermalised) start and momenta, and the
+    reversibility violations of both forms."""
     dims = (L, L, L, L, 4)
     out = []
     l64 = S.Lattice(S.SU2, dims, beta, precision=S.FP64)
     l32 = S.Lattice(S.SU2, dims, beta, precision=S.FP32)
-    l64.haarstart(seed)
-    # thermalise a little in FP64 so the configuration is not a hot start
-    for i in range(5):
+    l64.coldstart()
+    # thermalise in FP64 (cold start, plaquette relaxes within a few trajectories)
+    for i in range(ntherm):
         l64.md_update(seed + 100 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
     for i in range(ntraj):
         l32.set_links(l64.get_links())
@@ -100,15 +101,44 @@ def delta_dH(S, L, nsteps, beta, seed, ntraj):
         r32 = l32.md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE, revtest=True)
         out.append((r32["deltaH"] - r64["deltaH"], r32["deltadeltaH"], r64["deltadeltaH"],
                     r32["accept"] == r64["accept"]))
+    plaq = l64.gauge_energy() / (6 * L ** 4 * 2)
     l64.close()
     l32.close()
     return {
         "ntraj": ntraj,
+        "start": f"cold start + {ntherm} FP64 OMF2 trajectories, plaquette {plaq:.5f}",
         "max_abs_dH32_minus_dH64": max(abs(o[0]) for o in out),
         "max_abs_ddH_fp32": max(abs(o[1]) for o in out),
         "max_abs_ddH_fp64": max(abs(o[2]) for o in out),
         "same_accept_decisions": sum(o[3] for o in out),
     }
+
+
+def batch_ensemble(S, steps, warmup):
+    """The mixed ensemble of 224 lattices (128 U(1) d=2, 64 SU(2) d=3, 32 SU(2)
+    d=4), FP32, one launch per MD tick, hipGraph replay, autotuned launch shape."""
+    members = S.ensemble_members()
+    b = S.Batch(members, precision=S.FP32)
+    cache = os.path.join(ROOT, "gpurun_out", "batch_tune.json")
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    t0 = time.perf_counter()
+    blk, grd = b.autotune(cache)
+    t_tune = time.perf_counter() - t0
+    b.use_graph(True)
+    for i in range(warmup):
+        b.trajectory(10_000 + i)
+    acc = 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        res = b.trajectory(i)
+        acc += sum(r["accept"] for r in res)
+    dt = time.perf_counter() - t0
+    lu = b.link_updates()
+    b.close()
+    return {"lattices": len(members), "link_updates_per_traj": lu,
+            "link_updates_per_s": lu * steps / dt, "ms_per_traj": 1000 * dt / steps,
+            "acceptance": acc / (steps * len(members)), "block": blk, "grid": grd,
+            "autotune_s": round(t_tune, 3), "graph": True, "precision": "fp32"}
 
 
 def cpu_baseline(L, nsteps, beta, seed):
@@ -149,6 +179,8 @@ def main():
     ap.add_argument("--seed", type=int, default=13526463)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ddh-traj", type=int, default=3)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -160,6 +192,9 @@ def main():
     f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
     f64 = run_form(S, S.FP64, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
     ddh = delta_dH(S, args.L, args.nsteps, args.beta, args.seed, args.ddh_traj)
+    copy_ms = S.copy_probe(1 << 30, 20)
+    copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
@@ -197,6 +232,9 @@ def main():
                      fused_roofline_frac=f64["fused_achieved_GBs"] / HBM_PEAK_GBS),
         "fp32_over_fp64": f32["link_updates_per_s"] / f64["link_updates_per_s"],
         "delta_dH": ddh,
+        "measured_copy_GBs": copy_gbs,
+        "fused_frac_of_measured_copy_fp32": f32["fused_achieved_GBs"] / copy_gbs,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

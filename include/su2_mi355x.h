@@ -177,12 +177,78 @@ int su2gpu_runge_kutta(su2gpu_lattice *h, double eps);
 int su2gpu_gradient_flow(su2gpu_lattice *h, double tmax, double eps, double tstart, double *rows,
                          size_t max_rows, size_t *nrows);
 
+/* ---- gradient-flow adjoint (no reference counterpart: an addition over
+ * flat-gradient_flow.hh:17-50) ---------------------------------------------------
+ * su2gpu_flow_forward runs nsteps RK3 steps of size eps from the handle's links
+ * (left untouched), keeping a checkpoint every ckpt_every steps, and returns the
+ * observable at t = nsteps * eps:
+ *   SU2GPU_OBS_CLOVER_E      t^2 E(t), E the clover energy density of energy_density
+ *                            (flat-energy_density.hh:57-164);
+ *   SU2GPU_OBS_WILSON_ACTION S(t) = get_S_G_hmc (flat-gaugemonomial.hh:46-54), whose
+ *                            gradient at nsteps = 0 is the HMC force.
+ * su2gpu_flow_backward returns dO/dU of the last forward w.r.t. the links it
+ * started from, in the momentum layout and frame of gaugemonomial::derivative
+ * (U -> exp(X) U, dO = sum grad . x), recomputing the flow states of each
+ * checkpoint segment.  SU(2) only. */
+enum { SU2GPU_OBS_CLOVER_E = 0, SU2GPU_OBS_WILSON_ACTION = 1 };
+int su2gpu_flow_forward(su2gpu_lattice *h, double eps, size_t nsteps, size_t ckpt_every,
+                        int observable, double *value);
+int su2gpu_flow_backward(su2gpu_lattice *h, double *grad);
+
+/* ---- batched ensemble (no reference counterpart: many independent copies of
+ * the md_update of md_update.hh:18-81 advanced together) ---------------------------
+ * A batch holds lattices of uneven sizes (U(1) / SU(2), d = 2, 3, 4, own beta,
+ * integrator leapfrog or omf2, own step count and tau).  One trajectory of the
+ * batch is one launch per MD tick for all lattices (optionally replayed from a
+ * captured hipGraph), FP64 action / kinetic sums per lattice, and a Metropolis
+ * decision per lattice with the Philox uniform of (seed, traj).  Links start cold. */
+typedef struct su2gpu_batch su2gpu_batch;
+typedef struct {
+  int group;                       /* SU2GPU_U1 or SU2GPU_SU2 */
+  size_t Lx, Ly, Lz, Lt, ndims;    /* as su2gpu_create */
+  double beta;
+  int integrator;                  /* SU2GPU_LEAPFROG or SU2GPU_OMF2 */
+  size_t nsteps;
+  double tau;
+  uint64_t seed;                   /* key of the lattice's device momenta / accept uniform */
+} su2gpu_batch_member;
+int su2gpu_batch_create(su2gpu_batch **out, const su2gpu_batch_member *members, size_t n,
+                        int precision);
+int su2gpu_batch_destroy(su2gpu_batch *b);
+int su2gpu_batch_size(const su2gpu_batch *b, size_t *n);
+/* links / momenta of member i in the packed reference AoS order */
+int su2gpu_batch_set_links(su2gpu_batch *b, size_t i, const double *src);
+int su2gpu_batch_get_links(su2gpu_batch *b, size_t i, double *dst);
+int su2gpu_batch_set_momenta(su2gpu_batch *b, size_t i, const double *src);
+/* one trajectory of every member; momenta_source SU2GPU_MOMENTA_DEVICE_PHILOX
+ * draws them on the GPU from (member seed, traj), SU2GPU_MOMENTA_HOST_MT19937
+ * uses the ones set with su2gpu_batch_set_momenta.  metropolis = 0 accepts
+ * every trajectory.  results has one entry per member. */
+int su2gpu_batch_trajectory(su2gpu_batch *b, int momenta_source, uint64_t traj, int metropolis,
+                            su2gpu_md_result *results);
+/* launch shape: threads per tile/workgroup (64, 128, 256) and grid (0 = one pass) */
+int su2gpu_batch_set_launch(su2gpu_batch *b, int block, int grid);
+int su2gpu_batch_get_launch(const su2gpu_batch *b, int *block, int *grid);
+/* autotuner: times the candidate shapes on this batch, applies the fastest and
+ * records it in a JSON cache (cache_path, or $SU2GPU_TUNE_CACHE; NULL and unset:
+ * no cache); a cached entry is applied without timing */
+int su2gpu_batch_autotune(su2gpu_batch *b, const char *cache_path, int *block, int *grid);
+/* replay trajectories from a captured hipGraph (on) or launch them eagerly (off) */
+int su2gpu_batch_use_graph(su2gpu_batch *b, int on);
+/* sum over members of nd * V * nsteps: the link-updates of one batch trajectory */
+int su2gpu_batch_link_updates(const su2gpu_batch *b, double *per_trajectory);
+
 /* ---- configuration files (gaugeconfig.hh:225-252) ------------------------------ */
 int su2gpu_save(su2gpu_lattice *h, const char *path);
 int su2gpu_load(su2gpu_lattice *h, const char *path);
 
 /* ---- stream / timing ------------------------------------------------------------ */
 int su2gpu_synchronize(su2gpu_lattice *h);
+/* Measurement helper (no reference counterpart): times `reps` launches of a
+ * plain 16-byte-per-lane copy kernel of nbytes (read nbytes + write nbytes) and
+ * returns the average milliseconds per copy -- the attainable HBM bandwidth and
+ * the calibration of the hardware byte counters. */
+int su2gpu_copy_probe(size_t nbytes, int reps, double *ms_per_copy);
 /* HIP-event timing of every kernel launch on the handle's stream (off by
  * default).  su2gpu_profile_read(h, name, &ms, &count) returns the summed
  * duration and launch count of kernel class `name` ("md_fused", "md_p_only",

@@ -28,6 +28,7 @@
  * dims[] = {Lx, Ly, Lz, Lt, nd} everywhere (gaugeconfig.hh:49-63 argument
  * order); g = 0 for U(1), 1 for SU(2).
  */
+#define _GNU_SOURCE /* sincos */
 #include <complex.h>
 #include <math.h>
 #include <stdint.h>
@@ -257,9 +258,12 @@ static su2 random_su2(orc_mt19937 *e, double delta) {
   const double u = uniform(e, -1.0, 1.0);
   const double theta = uniform(e, 0.0, 2 * ORC_PI);
   const double r = sqrt(1 - u * u);
-  const double salpha = sin(alpha);
+  /* one sincos per angle, as the reference's GCC build evaluates the pairs */
+  double salpha, calpha, st, ct;
+  sincos(alpha, &salpha, &calpha);
+  sincos(theta, &st, &ct);
   const double sr = salpha * r;
-  su2 o = {cos(alpha), salpha * u, sr * sin(theta), sr * cos(theta)};
+  su2 o = {calpha, salpha * u, sr * st, sr * ct};
   return o;
 }
 static double random_u1(orc_mt19937 *e, double delta) {

@@ -23,6 +23,7 @@ LEAPFROG, LP_LEAPFROG, OMF4, LP_OMF4, EULER, RUTH, OMF2 = range(7)
 INTEGRATORS = {"leapfrog": LEAPFROG, "lp_leapfrog": LP_LEAPFROG, "omf4": OMF4,
                "lp_omf4": LP_OMF4, "euler": EULER, "ruth": RUTH, "omf2": OMF2}
 MOMENTA_HOST, MOMENTA_DEVICE = 0, 1
+OBS_CLOVER_E, OBS_WILSON_ACTION = 0, 1
 
 _lib = None
 
@@ -85,13 +86,39 @@ _SIGS = {
     "su2gpu_runge_kutta": (C.c_int, [_h, C.c_double]),
     "su2gpu_gradient_flow": (C.c_int, [_h, C.c_double, C.c_double, C.c_double, _vp, C.c_size_t,
                                        C.POINTER(C.c_size_t)]),
+    "su2gpu_flow_forward": (C.c_int, [_h, C.c_double, C.c_size_t, C.c_size_t, C.c_int, _dp]),
+    "su2gpu_flow_backward": (C.c_int, [_h, _vp]),
     "su2gpu_save": (C.c_int, [_h, C.c_char_p]),
     "su2gpu_load": (C.c_int, [_h, C.c_char_p]),
     "su2gpu_synchronize": (C.c_int, [_h]),
+    "su2gpu_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _dp]),
     "su2gpu_profile_enable": (C.c_int, [_h, C.c_int]),
     "su2gpu_profile_reset": (C.c_int, [_h]),
     "su2gpu_profile_read": (C.c_int, [_h, C.c_char_p, _dp, C.POINTER(C.c_long)]),
 }
+
+
+class BatchMember(C.Structure):
+    _fields_ = [("group", C.c_int), ("Lx", C.c_size_t), ("Ly", C.c_size_t), ("Lz", C.c_size_t),
+                ("Lt", C.c_size_t), ("ndims", C.c_size_t), ("beta", C.c_double),
+                ("integrator", C.c_int), ("nsteps", C.c_size_t), ("tau", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+_SIGS.update({
+    "su2gpu_batch_create": (C.c_int, [C.POINTER(_h), C.POINTER(BatchMember), C.c_size_t, C.c_int]),
+    "su2gpu_batch_destroy": (C.c_int, [_h]),
+    "su2gpu_batch_size": (C.c_int, [_h, C.POINTER(C.c_size_t)]),
+    "su2gpu_batch_set_links": (C.c_int, [_h, C.c_size_t, _vp]),
+    "su2gpu_batch_get_links": (C.c_int, [_h, C.c_size_t, _vp]),
+    "su2gpu_batch_set_momenta": (C.c_int, [_h, C.c_size_t, _vp]),
+    "su2gpu_batch_trajectory": (C.c_int, [_h, C.c_int, C.c_uint64, C.c_int, C.POINTER(MDResult)]),
+    "su2gpu_batch_set_launch": (C.c_int, [_h, C.c_int, C.c_int]),
+    "su2gpu_batch_get_launch": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "su2gpu_batch_autotune": (C.c_int, [_h, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "su2gpu_batch_use_graph": (C.c_int, [_h, C.c_int]),
+    "su2gpu_batch_link_updates": (C.c_int, [_h, _dp]),
+})
 
 
 def exported_symbols():
@@ -298,6 +325,17 @@ class Lattice:
                                           C.byref(n)))
         return rows[: n.value]
 
+    # -- gradient-flow adjoint
+    def flow_forward(self, eps, nsteps, ckpt_every=20, observable=OBS_CLOVER_E):
+        v = C.c_double()
+        _ck(self.lib.su2gpu_flow_forward(self._h, eps, nsteps, ckpt_every, observable, C.byref(v)))
+        return v.value
+
+    def flow_backward(self):
+        g = np.zeros(self.mom_shape())
+        _ck(self.lib.su2gpu_flow_backward(self._h, _ptr(g)))
+        return g
+
     # -- files
     def save(self, path):
         _ck(self.lib.su2gpu_save(self._h, path.encode()))
@@ -319,6 +357,125 @@ class Lattice:
         ms, cnt = C.c_double(), C.c_long()
         _ck(self.lib.su2gpu_profile_read(self._h, name.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+class Batch:
+    """Many independent lattices advanced together (su2gpu_batch_* of the C ABI).
+
+    members: sequence of dicts with keys group, dims=(Lx,Ly,Lz,Lt,nd), beta,
+    integrator ('leapfrog'|'omf2' or id), nsteps, tau, seed.
+    """
+
+    def __init__(self, members, precision=FP32):
+        lib = load_library()
+        arr = (BatchMember * len(members))()
+        self.members = []
+        for i, m in enumerate(members):
+            Lx, Ly, Lz, Lt, nd = m["dims"]
+            integ = m.get("integrator", LEAPFROG)
+            if isinstance(integ, str):
+                integ = INTEGRATORS[integ]
+            arr[i] = BatchMember(m["group"], Lx, Ly, Lz, Lt, nd, m["beta"], integ, m["nsteps"],
+                                 m.get("tau", 1.0), m.get("seed", 13526463 + 1000 * i))
+            self.members.append(dict(m, V=Lx * Ly * Lz * Lt, nd=nd))
+        h = C.c_void_p()
+        _ck(lib.su2gpu_batch_create(C.byref(h), arr, len(members), precision))
+        self._h, self.lib, self.n = h, lib, len(members)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.su2gpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _shape(self, i, mom=False):
+        m = self.members[i]
+        n = m["V"] * m["nd"]
+        if m["group"] == SU2:
+            return (n, 3 if mom else 4)
+        return (n,)
+
+    def set_links(self, i, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == self._shape(i)
+        _ck(self.lib.su2gpu_batch_set_links(self._h, i, _ptr(U)))
+
+    def get_links(self, i):
+        U = np.zeros(self._shape(i))
+        _ck(self.lib.su2gpu_batch_get_links(self._h, i, _ptr(U)))
+        return U
+
+    def set_momenta(self, i, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == self._shape(i, True)
+        _ck(self.lib.su2gpu_batch_set_momenta(self._h, i, _ptr(P)))
+
+    def trajectory(self, traj, momenta=MOMENTA_DEVICE, metropolis=True):
+        res = (MDResult * self.n)()
+        _ck(self.lib.su2gpu_batch_trajectory(self._h, momenta, traj, int(metropolis), res))
+        return [r.as_dict() for r in res]
+
+    def set_launch(self, block, grid=0):
+        _ck(self.lib.su2gpu_batch_set_launch(self._h, block, grid))
+
+    def get_launch(self):
+        b, g = C.c_int(), C.c_int()
+        _ck(self.lib.su2gpu_batch_get_launch(self._h, C.byref(b), C.byref(g)))
+        return b.value, g.value
+
+    def autotune(self, cache_path=None):
+        b, g = C.c_int(), C.c_int()
+        _ck(self.lib.su2gpu_batch_autotune(self._h, cache_path.encode() if cache_path else None,
+                                           C.byref(b), C.byref(g)))
+        return b.value, g.value
+
+    def use_graph(self, on=True):
+        _ck(self.lib.su2gpu_batch_use_graph(self._h, int(on)))
+
+    def link_updates(self):
+        v = C.c_double()
+        _ck(self.lib.su2gpu_batch_link_updates(self._h, C.byref(v)))
+        return v.value
+
+
+def ensemble_members(seed=13526463):
+    """The mixed ensemble: 128 U(1) d=2 lattices, L uniform in {16..128}, beta in [1,6];
+    64 SU(2) d=3, L in {8..24}, beta in [3,8]; 32 SU(2) d=4, L in {8,12,16}, beta in
+    [2.2,2.6]; leapfrog tau = 1 with 10..40 steps, seeded per lattice."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(128):
+        L = int(rng.integers(16, 129))
+        out.append(dict(group=U1, dims=(L, 1, 1, L, 2), beta=float(rng.uniform(1, 6))))
+    for _ in range(64):
+        L = int(rng.integers(8, 25))
+        out.append(dict(group=SU2, dims=(L, L, 1, L, 3), beta=float(rng.uniform(3, 8))))
+    for _ in range(32):
+        L = int(rng.choice([8, 12, 16]))
+        out.append(dict(group=SU2, dims=(L, L, L, L, 4), beta=float(rng.uniform(2.2, 2.6))))
+    for i, m in enumerate(out):
+        m.update(integrator=LEAPFROG, nsteps=int(rng.integers(10, 41)), tau=1.0,
+                 seed=seed + 1000 * i)
+    return out
+
+
+def copy_probe(nbytes, reps=20):
+    """Average ms of a plain device copy of nbytes (read + write nbytes each)."""
+    lib = load_library()
+    ms = C.c_double()
+    _ck(lib.su2gpu_copy_probe(int(nbytes), reps, C.byref(ms)))
+    return ms.value
 
 
 def device_count() -> int:

@@ -1,0 +1,43 @@
+// su2_batch.hpp -- descriptors and launchers of the batched-ensemble kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "su2_launch.hpp"
+
+namespace su2k {
+
+// one lattice of the batch (device copy)
+struct BLat {
+  Geo geo;
+  int group;      // 0 U(1), 1 SU(2)
+  int kind;       // integrator id: 0 leapfrog, 6 omf2
+  int npairs;     // number of "P then U" pairs; tick npairs is the final momentum step
+  int cur;        // arena holding the configuration at the start of the trajectory
+  int fin;        // arena holding the last link update
+  int tile0, ntiles;
+  long uoff;      // byte offset of the lattice's links in every link arena
+  long poff;      // element offset of its momenta in the momentum arena
+  double coef;    // beta / Nc
+  double dtau;    // tau / nsteps
+  uint64_t seed;  // Philox key of its momenta
+};
+
+struct BArenas {
+  char *U[3];
+  void *P;
+};
+
+void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
+                       int tick, int flags, double *partials, int grid, int block, hipStream_t st);
+void launch_batch_plaq(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
+                       int use_final, int row, double *partials, int grid, int block, hipStream_t st);
+void launch_batch_restore(int prec, const BLat *lats, const int2 *tiles, int ntiles,
+                          const BArenas &ar, int grid, int block, hipStream_t st);
+void launch_batch_momenta(int prec, const BLat *lats, const int2 *tiles, int ntiles,
+                          const BArenas &ar, const uint64_t *traj, int grid, int block,
+                          hipStream_t st);
+void launch_batch_reduce(const BLat *lats, int nlat, const double *partials, int ntiles,
+                         double *out, hipStream_t st);
+
+}  // namespace su2k

@@ -7,6 +7,7 @@
 // reference draws on the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -82,6 +83,21 @@ struct su2gpu_lattice {
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
   std::map<std::string, std::pair<double, long>> prof_acc;
+  // gradient-flow forward state kept for the backward pass
+  struct Flow {
+    bool valid = false;
+    double eps = 0, value = 0;
+    size_t nsteps = 0, ckpt = 0;
+    int obs = 0;
+    std::vector<void *> ckpts;          // W at steps 0, ckpt, 2 ckpt, ...
+    void *W[2] = {nullptr, nullptr};    // forward ping-pong; W[fin] = W(t)
+    int fin = 0;
+    void *Z = nullptr;                  // RK3 Z field
+    std::vector<void *> seg;            // recomputed W_k of one segment
+    void *W1 = nullptr, *W2 = nullptr;  // stage states of one step
+    void *Zs[2] = {nullptr, nullptr};   // Z after stages 0 and 1
+    void *lamW = nullptr, *lamZ = nullptr, *Gam = nullptr;
+  } flow;
 };
 
 namespace {
@@ -188,6 +204,27 @@ su2k::FusedCoef base_coef(const L *h) {
 }
 
 double *row(L *h, int r) { return h->partials + (size_t)r * h->row_stride; }
+
+// fused "P then U" (Uout != nullptr) or momentum-only step on raw buffers
+int fused_ptr(L *h, const void *Uin, void *Uout, void *P, su2k::FusedCoef c, int accum_row) {
+  su2k::FusedLaunch fl;
+  fl.Uin = Uin;
+  fl.Uout = Uout;
+  fl.P = P;
+  fl.geo = h->geo;
+  fl.c = c;
+  fl.partials = nullptr;
+  fl.pstride = h->row_stride;
+  if (accum_row >= 0) {
+    fl.c.flags |= su2k::F_ACCUM;
+    fl.partials = h->partials + (size_t)accum_row * h->row_stride;
+  }
+  {
+    Prof pr(h, Uout ? "md_fused" : "md_p_only");
+    su2k::launch_fused(h->group, h->prec, fl, h->stream);
+  }
+  return last_launch("md_fused");
+}
 
 // fused "P then U" (Uout >= 0) or momentum-only (Uout < 0) step
 int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row) {
@@ -418,12 +455,16 @@ void random_su2(std::mt19937 &e, double delta, double *q) {
   const double u = dist1(e);
   const double theta = dist2(e);
   const double r = std::sqrt(1 - u * u);
-  const double salpha = std::sin(alpha);
+  // the reference build evaluates each sin/cos pair with one glibc sincos call
+  // (GCC merges them); sincos and separate sin/cos can differ in the last bit
+  double salpha, calpha, st, ct;
+  ::sincos(alpha, &salpha, &calpha);
+  ::sincos(theta, &st, &ct);
   const double sr = salpha * r;
-  q[0] = std::cos(alpha);
+  q[0] = calpha;
   q[1] = salpha * u;
-  q[2] = sr * std::sin(theta);
-  q[3] = sr * std::cos(theta);
+  q[2] = sr * st;
+  q[3] = sr * ct;
 }
 double random_u1(std::mt19937 &e, double delta) {
   std::uniform_real_distribution<double> dist(-ref_pi() * delta, ref_pi() * delta);
@@ -529,6 +570,12 @@ extern "C" {
 
 const char *su2gpu_last_error(void) { return g_err.c_str(); }
 
+// error channel shared with the batch translation unit (not part of the header)
+int su2gpu_internal_set_error(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+
 int su2gpu_device_count(int *n) {
   if (!n) return fail(SU2GPU_EINVAL, "null pointer");
   HC(hipGetDeviceCount(n));
@@ -611,6 +658,13 @@ int su2gpu_destroy(su2gpu_lattice *h) {
     hipEventDestroy(r.b);
   }
   for (auto ev : h->evpool) hipEventDestroy(ev);
+  {
+    auto &f = h->flow;
+    for (void *p : f.ckpts) hipFree(p);
+    for (void *p : f.seg) hipFree(p);
+    for (void *p : {f.W[0], f.W[1], f.Z, f.W1, f.W2, f.Zs[0], f.Zs[1], f.lamW, f.lamZ, f.Gam})
+      if (p) hipFree(p);
+  }
   for (int i = 0; i < NBUF; ++i)
     if (h->U[i]) hipFree(h->U[i]);
   if (h->P) hipFree(h->P);
@@ -1134,6 +1188,176 @@ int su2gpu_gradient_flow(su2gpu_lattice *h, double tmax, double eps, double tsta
   return rc;
 }
 
+// ------------------------------------------------------ gradient-flow adjoint --
+namespace {
+
+int flow_alloc(L *h, void **p, size_t bytes) {
+  if (!*p) HC(hipMalloc(p, bytes));
+  return SU2GPU_OK;
+}
+
+// coefficients of one RK3 step (flat-gradient_flow.hh:17-50): stage f adds
+// b_f D(W) to Z (b_f = fac_f * beta / Nc with fac_f = 2 Nc z_f / beta) and
+// multiplies W by exp(c_f Z), c_f = -eps * expfac_f
+void rk_coef(const L *h, double eps, double b[3], double c[3]) {
+  const double zfac[3] = {(-17.0) / (36.0), (8.0) / (9.0), (-3.0) / (4.0)};
+  const double expfac[3] = {-36.0 / 4. / 17.0, 1., -1.};
+  for (int f = 0; f < 3; ++f) {
+    const double fac = 2. * Nc(h) * zfac[f] / h->beta;
+    b[f] = fac * h->beta / Nc(h);
+    c[f] = -eps * expfac[f];
+  }
+}
+
+// one RK3 stage on raw buffers: Z (+)= b D(Win); Wout = exp(c Z) Win (Wout may be null)
+int rk_stage(L *h, const void *Win, void *Wout, void *Z, double b, double c, bool first) {
+  su2k::FusedCoef k = base_coef(h);
+  if (h->prec == 1 && h->group == 1) k.flags |= su2k::F_RESTORE;
+  k.coef = b;
+  k.ep = -1.0;
+  k.eu = c;
+  if (first) k.flags |= su2k::F_PZERO;
+  return fused_ptr(h, Win, Wout, Z, k, -1);
+}
+
+// full RK3 step Win -> Wout using tmp as the intermediate buffer
+int rk_step(L *h, const void *Win, void *Wout, void *tmp, void *Z, double eps) {
+  double b[3], c[3];
+  rk_coef(h, eps, b, c);
+  CK(rk_stage(h, Win, Wout, Z, b[0], c[0], true));
+  CK(rk_stage(h, Wout, tmp, Z, b[1], c[1], false));
+  CK(rk_stage(h, tmp, Wout, Z, b[2], c[2], false));
+  return SU2GPU_OK;
+}
+
+}  // namespace
+
+int su2gpu_flow_forward(su2gpu_lattice *h, double eps, size_t nsteps, size_t ckpt_every,
+                        int observable, double *value) {
+  CK(check(h));
+  if (!value) return fail(SU2GPU_EINVAL, "null value");
+  if (h->group != SU2GPU_SU2) return fail(SU2GPU_EINVAL, "the flow adjoint is implemented for SU(2)");
+  if (observable != SU2GPU_OBS_CLOVER_E && observable != SU2GPU_OBS_WILSON_ACTION)
+    return fail(SU2GPU_EINVAL, "bad observable");
+  if (ckpt_every < 1) return fail(SU2GPU_EINVAL, "ckpt_every must be >= 1");
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
+  auto &f = h->flow;
+  const size_t nck = (nsteps + ckpt_every - 1) / ckpt_every;
+  for (void *p : f.ckpts) HC(hipFree(p));
+  f.ckpts.assign(nck, nullptr);
+  for (auto &p : f.ckpts) HC(hipMalloc(&p, h->U_bytes));
+  CK(flow_alloc(h, &f.W[0], h->U_bytes));
+  CK(flow_alloc(h, &f.W[1], h->U_bytes));
+  CK(flow_alloc(h, &f.W1, h->U_bytes));
+  CK(flow_alloc(h, &f.Z, h->P_bytes));
+  HC(hipMemcpyAsync(f.W[0], h->U[h->cur], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+  int a = 0;
+  for (size_t k = 0; k < nsteps; ++k) {
+    if (k % ckpt_every == 0)
+      HC(hipMemcpyAsync(f.ckpts[k / ckpt_every], f.W[a], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+    CK(rk_step(h, f.W[a], f.W[1 - a], f.W1, f.Z, eps));
+    a = 1 - a;
+  }
+  f.fin = a;
+  f.eps = eps;
+  f.nsteps = nsteps;
+  f.ckpt = ckpt_every;
+  f.obs = observable;
+  // observable at W(t): t^2 E_clover(t) or the Wilson action S (get_S_G_hmc)
+  const int save = h->cur;
+  int tmp;
+  CK(free_buf(h, {save}, &tmp));
+  HC(hipMemcpyAsync(h->U[tmp], f.W[a], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+  h->cur = tmp;
+  int rc;
+  const double t = eps * (double)nsteps;
+  if (observable == SU2GPU_OBS_CLOVER_E) {
+    double E, Q;
+    rc = su2gpu_energy_density(h, 1, 0, &E, &Q);
+    f.value = t * t * E;
+  } else {
+    double S;
+    rc = su2gpu_gauge_action(h, &S);
+    f.value = S;
+  }
+  h->cur = save;
+  CK(rc);
+  f.valid = true;
+  *value = f.value;
+  return SU2GPU_OK;
+}
+
+int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
+  CK(check(h));
+  if (!grad) return fail(SU2GPU_EINVAL, "null grad");
+  auto &f = h->flow;
+  if (!f.valid) return fail(SU2GPU_ESTATE, "no flow_forward to differentiate");
+  CK(flow_alloc(h, &f.W2, h->U_bytes));
+  CK(flow_alloc(h, &f.Zs[0], h->P_bytes));
+  CK(flow_alloc(h, &f.Zs[1], h->P_bytes));
+  CK(flow_alloc(h, &f.lamW, h->P_bytes));
+  CK(flow_alloc(h, &f.lamZ, h->P_bytes));
+  if (f.seg.size() < f.ckpt) {
+    size_t old = f.seg.size();
+    f.seg.resize(f.ckpt, nullptr);
+    for (size_t i = old; i < f.ckpt; ++i) HC(hipMalloc(&f.seg[i], h->U_bytes));
+  }
+  const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
+  // cotangent of W(t)
+  const double t = f.eps * (double)f.nsteps;
+  if (f.obs == SU2GPU_OBS_CLOVER_E) {
+    CK(flow_alloc(h, &f.Gam, (size_t)h->V * 6 * 3 * h->real_bytes));
+    {
+      Prof pr(h, "flow_clover_grad");
+      su2k::launch_clover_grad(h->prec, f.W[f.fin], f.Gam, f.lamW, h->geo,
+                               t * t / (4.0 * (double)h->V), h->stream);
+    }
+    CK(last_launch("clover_grad"));
+  } else {
+    HC(hipMemsetAsync(f.lamW, 0, h->P_bytes, h->stream));
+    su2k::FusedCoef k = base_coef(h);
+    k.ep = -1.0;  // lamW += (beta/Nc) D(W) = grad S
+    CK(fused_ptr(h, f.W[f.fin], nullptr, f.lamW, k, -1));
+  }
+  double b[3], c[3];
+  rk_coef(h, f.eps, b, c);
+  const size_t nseg = f.ckpts.size();
+  for (size_t sg = nseg; sg-- > 0;) {
+    const size_t k0 = sg * f.ckpt, k1 = std::min(f.nsteps, k0 + f.ckpt);
+    // recompute W_k, k0 <= k < k1, from the checkpoint
+    HC(hipMemcpyAsync(f.seg[0], f.ckpts[sg], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+    for (size_t k = k0; k + 1 < k1; ++k)
+      CK(rk_step(h, f.seg[k - k0], f.seg[k - k0 + 1], f.W1, f.Z, f.eps));
+    for (size_t k = k1; k-- > k0;) {
+      const void *W0 = f.seg[k - k0];
+      // stage states: Z0' = b0 D(W0), W1 = exp(c0 Z0') W0, Z1' = Z0' + b1 D(W1),
+      // W2 = exp(c1 Z1') W1, Z2' = Z1' + b2 D(W2)
+      CK(rk_stage(h, W0, f.W1, f.Z, b[0], c[0], true));
+      HC(hipMemcpyAsync(f.Zs[0], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+      CK(rk_stage(h, f.W1, f.W2, f.Z, b[1], c[1], false));
+      HC(hipMemcpyAsync(f.Zs[1], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+      CK(rk_stage(h, f.W2, nullptr, f.Z, b[2], c[2], false));
+      const void *Wst[3] = {W0, f.W1, f.W2};
+      const void *Zst[3] = {f.Zs[0], f.Zs[1], f.Z};
+      for (int st = 2; st >= 0; --st) {
+        {
+          Prof pr(h, "flow_stage_adj");
+          su2k::launch_flow_stage_adj(h->prec, Zst[st], f.lamW, f.lamZ, h->V, h->nd, c[st],
+                                      st == 2 ? 1 : 0, h->stream);
+        }
+        CK(last_launch("flow_stage_adj"));
+        {
+          Prof pr(h, "flow_vjp");
+          su2k::launch_flow_vjp(h->prec, Wst[st], f.lamZ, f.lamW, h->geo, b[st], w_t, w_s,
+                                h->stream);
+        }
+        CK(last_launch("flow_vjp"));
+      }
+    }
+  }
+  return get_mom_host(h, f.lamW, grad);
+}
+
 int su2gpu_save(su2gpu_lattice *h, const char *path) {
   CK(check(h));
   if (!path) return fail(SU2GPU_EINVAL, "null path");
@@ -1160,6 +1384,40 @@ int su2gpu_load(su2gpu_lattice *h, const char *path) {
   if (r != buf.size()) return fail(SU2GPU_EIO, "%s: expected %zu bytes, read %zu", path,
                                    buf.size() * sizeof(double), r * sizeof(double));
   return su2gpu_set_links(h, buf.data(), SU2GPU_LAYOUT_OBJECT);
+}
+
+int su2gpu_copy_probe(size_t nbytes, int reps, double *ms_per_copy) {
+  if (!ms_per_copy || nbytes < 16 || reps < 1) return fail(SU2GPU_EINVAL, "bad copy probe arguments");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const long n16 = (long)(nbytes / 16);
+  int rc = SU2GPU_OK;
+  hipError_t e = hipMalloc(&a, (size_t)n16 * 16);
+  if (e == hipSuccess) e = hipMalloc(&b, (size_t)n16 * 16);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 1, (size_t)n16 * 16, st);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    su2k::launch_copy16(a, b, n16, st);  // warm-up
+    e = hipEventRecord(e0, st);
+  }
+  if (e == hipSuccess) {
+    for (int r = 0; r < reps; ++r) su2k::launch_copy16(a, b, n16, st);
+    e = hipEventRecord(e1, st);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e != hipSuccess) rc = fail(SU2GPU_EHIP, "copy probe: %s", hipGetErrorString(e));
+  else *ms_per_copy = ms / reps;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
 }
 
 int su2gpu_synchronize(su2gpu_lattice *h) {

@@ -65,6 +65,14 @@ template <typename T> __device__ __forceinline__ quat<T> qrestore(const quat<T> 
   return quat<T>{u.ar / r, u.ai / r, u.br / r, u.bi / r};
 }
 
+// promotion of stored links to FP64 for the FP64 sums of the FP32 form
+__device__ __forceinline__ quat<double> to_d(const quat<float> &q) {
+  return quat<double>{double(q.ar), double(q.ai), double(q.br), double(q.bi)};
+}
+__device__ __forceinline__ quat<double> to_d(const quat<double> &q) { return q; }
+__device__ __forceinline__ double to_d(float x) { return double(x); }
+__device__ __forceinline__ double to_d(double x) { return x; }
+
 // error-free transformation a + b = s + e (Knuth TwoSum)
 template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T &e) {
   s = a + b;

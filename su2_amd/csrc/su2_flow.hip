@@ -1,0 +1,357 @@
+// su2_flow.hip -- adjoint (backward) kernels through the Wilson gradient flow
+// (Luescher RK3 of flat-gradient_flow.hh:17-50) for SU(2).
+//
+// Conventions.  A link perturbation is left-trivialised, U -> exp(X) U with X a
+// pure quaternion whose vector part is the adjointsu2 triple (a, b, c) (the
+// same frame as get_deriv, adjointfield.hh:53-56, and exp, exp_gauge.cc:8-18).
+// A cotangent is a triple per link lambda with dO = sum lambda . x.
+//
+// One RK3 stage is  Z' = a Z + b D(W),  W' = exp(c Z') W,  D(W) = 2 Im(W K)
+// (the force without beta/Nc).  Its adjoint:
+//   lambda_Z' += c J(cZ')^T lambda_W'      J(A) = sum ad_A^k/(k+1)!, J^T = J(-A)
+//   lambda_W   = Ad(exp(cZ'))^T lambda_W'  (= Ad(exp(-cZ')))
+//   lambda_W  += b (dD/dW)^T lambda_Z'     (Hessian-vector product, k_flow_vjp)
+//   lambda_Z   = a lambda_Z'
+//
+// (dD/dW)^T lambda is gathered per link from its 2(d-1) plaquettes.  For the
+// plaquette P = a b c^+ d^+ (a = U_mu(x), b = U_nu(x+mu), c = U_mu(x+nu),
+// d = U_nu(x)) sum_m lambda_m . D_m restricted to P equals -2 Re(P Omega) with
+// Omega = L_a + a L_b a^+ - d L_c d^+ - L_d (L = pure quaternion of lambda), and
+// the gradient of -Re(P Omega) w.r.t. the four links is
+//   a: Im(P Om) + Im([a L_b a^+, P])     b: Im(a^+ P Om a)
+//   c: -Im(d^+ Om P d)                   d: -Im(Om P) + Im([-d L_c d^+, P])
+// The clover energy density E = sum_{x,mu<nu} |Im C_{mu nu}(x)|^2 / (8V)
+// (flat-energy_density.hh:57-164 for SU(2)) has the same structure with the
+// insertions Gamma = Im C at the four corners of each plaquette:
+//   Om = G_x + a G_{x+mu} a^+ + (ab) G_{x+mu+nu} (ab)^+ + d G_{x+nu} d^+.
+#include "su2_device.hpp"
+#include "su2_launch.hpp"
+
+using namespace su2dev;
+
+namespace su2k {
+
+constexpr int NTF = 256;
+
+template <typename T> struct v3 {
+  T x, y, z;  // adjointsu2 (a, b, c)
+};
+
+// (a,b,c) <-> pure quaternion in storage order (Re a, Im a, Re b, Im b) = (q0, q3, q2, q1)
+template <typename T> __device__ __forceinline__ quat<T> vq(const v3<T> &v) {
+  return quat<T>{T(0), v.z, v.y, v.x};
+}
+template <typename T> __device__ __forceinline__ v3<T> im(const quat<T> &q) {
+  return v3<T>{q.bi, q.br, q.ai};
+}
+template <typename T> __device__ __forceinline__ quat<T> qsub(const quat<T> &u, const quat<T> &v) {
+  return quat<T>{u.ar - v.ar, u.ai - v.ai, u.br - v.br, u.bi - v.bi};
+}
+template <typename T> __device__ __forceinline__ quat<T> comm(const quat<T> &u, const quat<T> &v) {
+  return qsub(qmul(u, v), qmul(v, u));
+}
+template <typename T> __device__ __forceinline__ quat<T> conj_by(const quat<T> &g, const quat<T> &y) {
+  return qmul(qmul(g, y), qdag(g));  // g y g^+
+}
+template <typename T> __device__ __forceinline__ void vadd(v3<T> &a, const v3<T> &b, T s = T(1)) {
+  a.x += s * b.x;
+  a.y += s * b.y;
+  a.z += s * b.z;
+}
+template <typename T> __device__ __forceinline__ v3<T> ld3(const T *P, int plane, int V, int s) {
+  return v3<T>{P[(plane * 3 + 0) * V + s], P[(plane * 3 + 1) * V + s], P[(plane * 3 + 2) * V + s]};
+}
+
+// ---------------------------------------------------------------------------
+// pointwise part of one stage adjoint (see header), in place on lamW / lamZ
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NTF) void k_flow_stage_adj(const T *__restrict__ Zs, T *__restrict__ lamW,
+                                                        T *__restrict__ lamZ, int V, int nd,
+                                                        double c, int zero_lamZ) {
+  const int i = blockIdx.x * NTF + threadIdx.x;
+  if (i >= V * nd) return;
+  const int mu = i / V, s = i - mu * V;
+  const v3<T> z = ld3(Zs, mu, V, s);
+  const v3<T> y = ld3(lamW, mu, V, s);
+  const v3<T> a{T(c) * z.x, T(c) * z.y, T(c) * z.z};
+  const quat<T> A = vq(a), Y = vq(y);
+  // J(-A) y = y - f1 [A, y] + f2 [A, [A, y]],  theta = 2|a| (|ad_A| = 2|a|)
+  const T al2 = a.x * a.x + a.y * a.y + a.z * a.z;
+  const T th2 = T(4) * al2;
+  T f1, f2;
+  if (th2 < T(1e-6)) {
+    f1 = T(0.5) - th2 * T(1.0 / 24.0);
+    f2 = T(1.0 / 6.0) - th2 * T(1.0 / 120.0);
+  } else {
+    const T th = dsqrt(th2);
+    T sn, cs;
+    dsincos(th, &sn, &cs);
+    f1 = (T(1) - cs) / th2;
+    f2 = (th - sn) / (th2 * th);
+  }
+  const quat<T> k1 = comm(A, Y);
+  const quat<T> k2 = comm(A, k1);
+  const v3<T> jy = im(quat<T>{T(0), Y.ai - f1 * k1.ai + f2 * k2.ai, Y.br - f1 * k1.br + f2 * k2.br,
+                              Y.bi - f1 * k1.bi + f2 * k2.bi});
+  v3<T> lz = zero_lamZ ? v3<T>{0, 0, 0} : ld3(lamZ, mu, V, s);
+  vadd(lz, jy, T(c));
+  // Ad(exp(A))^T y = g^+ y g
+  const quat<T> g = qexp(a.x, a.y, a.z);
+  const v3<T> yw = im(qmul(qmul(qdag(g), Y), g));
+  lamZ[(mu * 3 + 0) * V + s] = lz.x;
+  lamZ[(mu * 3 + 1) * V + s] = lz.y;
+  lamZ[(mu * 3 + 2) * V + s] = lz.z;
+  lamW[(mu * 3 + 0) * V + s] = yw.x;
+  lamW[(mu * 3 + 1) * V + s] = yw.y;
+  lamW[(mu * 3 + 2) * V + s] = yw.z;
+}
+
+// role-specific gradient of -Re(P Omega) for the HVP insertion pattern
+template <typename T>
+__device__ __forceinline__ v3<T> hvp_role(int role, const quat<T> &a, const quat<T> &b,
+                                          const quat<T> &c, const quat<T> &d, const v3<T> &La,
+                                          const v3<T> &Lb, const v3<T> &Lc, const v3<T> &Ld) {
+  const quat<T> P = qmul(qmul(qmul(a, b), qdag(c)), qdag(d));
+  const quat<T> Tb = conj_by(a, vq(Lb));
+  const quat<T> Tc = qscale(T(-1), conj_by(d, vq(Lc)));
+  const quat<T> Om = qsub(qadd(qadd(vq(La), Tb), Tc), vq(Ld));
+  switch (role) {
+    case 0: {  // a
+      v3<T> r = im(qmul(P, Om));
+      vadd(r, im(comm(Tb, P)));
+      return r;
+    }
+    case 1:  // b
+      return im(qmul(qmul(qmul(qdag(a), P), Om), a));
+    case 2: {  // c
+      v3<T> r = im(qmul(qmul(qmul(qdag(d), Om), P), d));
+      return v3<T>{-r.x, -r.y, -r.z};
+    }
+    default: {  // d
+      v3<T> r = im(qmul(Om, P));
+      r = v3<T>{-r.x, -r.y, -r.z};
+      vadd(r, im(comm(Tc, P)));
+      return r;
+    }
+  }
+}
+
+// role-specific gradient of -Re(P Omega_E) for the clover insertions
+template <typename T>
+__device__ __forceinline__ v3<T> clover_role(int role, const quat<T> &a, const quat<T> &b,
+                                             const quat<T> &c, const quat<T> &d, const v3<T> &Gx,
+                                             const v3<T> &Gxm, const v3<T> &Gxmn, const v3<T> &Gxn) {
+  const quat<T> P = qmul(qmul(qmul(a, b), qdag(c)), qdag(d));
+  const quat<T> ab = qmul(a, b);
+  const quat<T> Tb = conj_by(a, vq(Gxm));
+  const quat<T> Tc = conj_by(ab, vq(Gxmn));
+  const quat<T> Td = conj_by(d, vq(Gxn));
+  const quat<T> Om = qadd(qadd(qadd(vq(Gx), Tb), Tc), Td);
+  switch (role) {
+    case 0: {
+      v3<T> r = im(qmul(P, Om));
+      vadd(r, im(comm(Tb, P)));
+      vadd(r, im(comm(Tc, P)));
+      return r;
+    }
+    case 1: {
+      const quat<T> aPa = qmul(qmul(qdag(a), P), a);
+      v3<T> r = im(qmul(qmul(qmul(qdag(a), P), Om), a));
+      vadd(r, im(comm(conj_by(b, vq(Gxmn)), aPa)));
+      return r;
+    }
+    case 2: {
+      v3<T> r = im(qmul(qmul(qmul(qdag(d), Om), P), d));
+      return v3<T>{-r.x, -r.y, -r.z};
+    }
+    default: {
+      v3<T> r = im(qmul(Om, P));
+      r = v3<T>{-r.x, -r.y, -r.z};
+      vadd(r, im(comm(Td, P)));
+      return r;
+    }
+  }
+}
+
+__device__ __forceinline__ int plane_id(int mu, int nu) {
+  // mu < nu: (0,1)=0 (0,2)=1 (0,3)=2 (1,2)=3 (1,3)=4 (2,3)=5
+  return mu == 0 ? nu - 1 : (mu == 1 ? nu + 1 : 5);
+}
+
+// lamW += bcoef * (dD/dW)^T lamZ   (per site, all link directions)
+template <typename T, int ND>
+__global__ __launch_bounds__(NTF) void k_flow_vjp(const quat<T> *__restrict__ U,
+                                                  const T *__restrict__ lamZ, T *__restrict__ lamW,
+                                                  Geo geo, double bcoef, double w_t, double w_s) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NTF + threadIdx.x;
+  const int V = geo.V;
+  if (s >= V) return;
+  int up[ND], dn[ND];
+  neighbour_offsets<ND>(geo, s, up, dn);
+  auto Ul = [&](int d, int site) { return U[d * V + site]; };
+  auto Ll = [&](int d, int site) { return ld3(lamZ, d, V, site); };
+#pragma unroll
+  for (int rho = 0; rho < ND; ++rho) {
+    v3<T> g{0, 0, 0};
+#pragma unroll
+    for (int sg = 0; sg < ND; ++sg) {
+      if (sg == rho) continue;
+      const T w = T((rho == 0 || sg == 0) ? w_t : w_s);
+      const int sm = s + dn[sg];  // s - sigma
+      v3<T> gi, gii;
+      if (rho < sg) {
+        // x = s (mu = rho, nu = sg): role a;  x = s - sg: role c
+        gi = hvp_role<T>(0, Ul(rho, s), Ul(sg, s + up[rho]), Ul(rho, s + up[sg]), Ul(sg, s),
+                         Ll(rho, s), Ll(sg, s + up[rho]), Ll(rho, s + up[sg]), Ll(sg, s));
+        gii = hvp_role<T>(2, Ul(rho, sm), Ul(sg, sm + up[rho]), Ul(rho, s), Ul(sg, sm), Ll(rho, sm),
+                          Ll(sg, sm + up[rho]), Ll(rho, s), Ll(sg, sm));
+      } else {
+        // x = s (mu = sg, nu = rho): role d;  x = s - sg: role b
+        gi = hvp_role<T>(3, Ul(sg, s), Ul(rho, s + up[sg]), Ul(sg, s + up[rho]), Ul(rho, s),
+                         Ll(sg, s), Ll(rho, s + up[sg]), Ll(sg, s + up[rho]), Ll(rho, s));
+        gii = hvp_role<T>(1, Ul(sg, sm), Ul(rho, s), Ul(sg, sm + up[rho]), Ul(rho, sm), Ll(sg, sm),
+                          Ll(rho, s), Ll(sg, sm + up[rho]), Ll(rho, sm));
+      }
+      vadd(g, gi, w);
+      vadd(g, gii, w);
+    }
+    const T f = T(2.0 * bcoef);
+    lamW[(rho * 3 + 0) * V + s] += f * g.x;
+    lamW[(rho * 3 + 1) * V + s] += f * g.y;
+    lamW[(rho * 3 + 2) * V + s] += f * g.z;
+  }
+}
+
+// pass 1 of grad E: Gamma_{mu nu}(x) = Im C_{mu nu}(x) (clover, flat-energy_density.hh:85-119)
+template <typename T, int ND>
+__global__ __launch_bounds__(NTF) void k_clover_im(const quat<T> *__restrict__ U, T *__restrict__ Gam,
+                                                   Geo geo) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NTF + threadIdx.x;
+  const int V = geo.V;
+  if (s >= V) return;
+  int up[ND], dn[ND];
+  neighbour_offsets<ND>(geo, s, up, dn);
+  auto L = [&](int site, int d) { return U[d * V + site]; };
+#pragma unroll
+  for (int mu = 0; mu < ND - 1; ++mu)
+#pragma unroll
+    for (int nu = mu + 1; nu < ND; ++nu) {
+      const int x1 = s + up[mu], x2 = s + up[nu];
+      quat<T> leaf = qmul(qmul(qmul(L(s, mu), L(x1, nu)), qdag(L(x2, mu))), qdag(L(s, nu)));
+      const int a1 = s + dn[mu] + up[nu], a2 = s + dn[mu];
+      leaf = qadd(leaf, qmul(qmul(qmul(L(s, nu), qdag(L(a1, mu))), qdag(L(a2, nu))), L(a2, mu)));
+      const int b2 = s + dn[mu] + dn[nu], b3 = s + dn[nu];
+      leaf = qadd(leaf, qmul(qmul(qmul(qdag(L(a2, mu)), qdag(L(b2, nu))), L(b2, mu)), L(b3, nu)));
+      const int c2 = s + dn[nu] + up[mu];
+      leaf = qadd(leaf, qmul(qmul(qmul(qdag(L(b3, nu)), L(b3, mu)), L(c2, nu)), qdag(L(s, mu))));
+      const int p = plane_id(mu, nu);
+      const v3<T> g = im(leaf);
+      Gam[(p * 3 + 0) * V + s] = g.x;
+      Gam[(p * 3 + 1) * V + s] = g.y;
+      Gam[(p * 3 + 2) * V + s] = g.z;
+    }
+}
+
+// pass 2 of grad E: lamW = scale * sum over the plaquettes of each link
+template <typename T, int ND>
+__global__ __launch_bounds__(NTF) void k_clover_grad(const quat<T> *__restrict__ U,
+                                                     const T *__restrict__ Gam, T *__restrict__ lamW,
+                                                     Geo geo, double scale) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NTF + threadIdx.x;
+  const int V = geo.V;
+  if (s >= V) return;
+  int up[ND], dn[ND];
+  neighbour_offsets<ND>(geo, s, up, dn);
+  auto Ul = [&](int d, int site) { return U[d * V + site]; };
+  auto Gl = [&](int m, int n, int site) { return ld3(Gam, plane_id(m, n), V, site); };
+#pragma unroll
+  for (int rho = 0; rho < ND; ++rho) {
+    v3<T> g{0, 0, 0};
+#pragma unroll
+    for (int sg = 0; sg < ND; ++sg) {
+      if (sg == rho) continue;
+      const int sm = s + dn[sg];
+      if (rho < sg) {
+        const int m = rho, n = sg;
+        // x = s: corners s, s+rho, s+rho+sg, s+sg; link is a
+        vadd(g, clover_role<T>(0, Ul(m, s), Ul(n, s + up[m]), Ul(m, s + up[n]), Ul(n, s), Gl(m, n, s),
+                               Gl(m, n, s + up[m]), Gl(m, n, s + up[m] + up[n]), Gl(m, n, s + up[n])));
+        // x = s - sg: corners sm, sm+rho, s+rho, s; link is c
+        vadd(g, clover_role<T>(2, Ul(m, sm), Ul(n, sm + up[m]), Ul(m, s), Ul(n, sm), Gl(m, n, sm),
+                               Gl(m, n, sm + up[m]), Gl(m, n, s + up[m]), Gl(m, n, s)));
+      } else {
+        const int m = sg, n = rho;
+        // x = s (mu = sg, nu = rho): corners s, s+sg, s+sg+rho, s+rho; link is d
+        vadd(g, clover_role<T>(3, Ul(m, s), Ul(n, s + up[m]), Ul(m, s + up[n]), Ul(n, s), Gl(m, n, s),
+                               Gl(m, n, s + up[m]), Gl(m, n, s + up[m] + up[n]), Gl(m, n, s + up[n])));
+        // x = s - sg: corners sm, s, s+rho, sm+rho; link is b
+        vadd(g, clover_role<T>(1, Ul(m, sm), Ul(n, s), Ul(m, sm + up[n]), Ul(n, sm), Gl(m, n, sm),
+                               Gl(m, n, s), Gl(m, n, s + up[n]), Gl(m, n, sm + up[n])));
+      }
+    }
+    const T f = T(scale);
+    lamW[(rho * 3 + 0) * V + s] = f * g.x;
+    lamW[(rho * 3 + 1) * V + s] = f * g.y;
+    lamW[(rho * 3 + 2) * V + s] = f * g.z;
+  }
+}
+
+// ------------------------------------------------------------- launchers ----
+static inline int nblkf(long n) { return (int)((n + NTF - 1) / NTF); }
+
+void launch_flow_stage_adj(int prec, const void *Zs, void *lamW, void *lamZ, int V, int nd, double c,
+                           int zero_lamZ, hipStream_t st) {
+  dim3 grid(nblkf((long)V * nd));
+  if (prec == 0)
+    hipLaunchKernelGGL((k_flow_stage_adj<double>), grid, dim3(NTF), 0, st, (const double *)Zs,
+                       (double *)lamW, (double *)lamZ, V, nd, c, zero_lamZ);
+  else
+    hipLaunchKernelGGL((k_flow_stage_adj<float>), grid, dim3(NTF), 0, st, (const float *)Zs,
+                       (float *)lamW, (float *)lamZ, V, nd, c, zero_lamZ);
+}
+
+template <typename T>
+static void vjp_t(const void *U, const void *lamZ, void *lamW, const Geo &g, double b, double w_t,
+                  double w_s, hipStream_t st) {
+  dim3 grid(nblkf(g.V));
+  switch (g.nd) {
+    case 2: hipLaunchKernelGGL((k_flow_vjp<T, 2>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)lamZ, (T *)lamW, g, b, w_t, w_s); break;
+    case 3: hipLaunchKernelGGL((k_flow_vjp<T, 3>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)lamZ, (T *)lamW, g, b, w_t, w_s); break;
+    default: hipLaunchKernelGGL((k_flow_vjp<T, 4>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)lamZ, (T *)lamW, g, b, w_t, w_s); break;
+  }
+}
+void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, const Geo &g, double b,
+                     double w_t, double w_s, hipStream_t st) {
+  if (prec == 0) vjp_t<double>(U, lamZ, lamW, g, b, w_t, w_s, st);
+  else vjp_t<float>(U, lamZ, lamW, g, b, w_t, w_s, st);
+}
+
+template <typename T>
+static void clover_t(const void *U, void *Gam, void *lamW, const Geo &g, double scale, hipStream_t st) {
+  dim3 grid(nblkf(g.V));
+  switch (g.nd) {
+    case 2:
+      hipLaunchKernelGGL((k_clover_im<T, 2>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (T *)Gam, g);
+      hipLaunchKernelGGL((k_clover_grad<T, 2>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)Gam, (T *)lamW, g, scale);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_clover_im<T, 3>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (T *)Gam, g);
+      hipLaunchKernelGGL((k_clover_grad<T, 3>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)Gam, (T *)lamW, g, scale);
+      break;
+    default:
+      hipLaunchKernelGGL((k_clover_im<T, 4>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (T *)Gam, g);
+      hipLaunchKernelGGL((k_clover_grad<T, 4>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)Gam, (T *)lamW, g, scale);
+      break;
+  }
+}
+void launch_clover_grad(int prec, const void *U, void *Gam, void *lamW, const Geo &g, double scale,
+                        hipStream_t st) {
+  if (prec == 0) clover_t<double>(U, Gam, lamW, g, scale, st);
+  else clover_t<float>(U, Gam, lamW, g, scale, st);
+}
+
+}  // namespace su2k

@@ -12,6 +12,7 @@
 // Sites s follow the reference's lexicographic order (gaugeconfig.hh:211-222).
 #include "su2_device.hpp"
 #include "su2_launch.hpp"
+#include "su2_md_site.hpp"
 
 using namespace su2dev;
 
@@ -36,69 +37,11 @@ __global__ __launch_bounds__(NT) void k_md_fused(const typename G::link *__restr
                                                  typename G::real *__restrict__ P, Geo geo,
                                                  FusedCoef c, double *__restrict__ partials,
                                                  int pstride) {
-  using T = typename G::real;
-  using link = typename G::link;
-  using acc = typename G::acc;
-  constexpr int NM = G::NMOM;
   __shared__ double lds[NT / 64];
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int s = lb * NT + threadIdx.x;
-  const int V = geo.V;
   double kin_old = 0.0, act = 0.0, kin_new = 0.0;
-  if (s < V) {
-    int up[ND], dn[ND];
-    neighbour_offsets<ND>(geo, s, up, dn);
-    link own[ND];
-#pragma unroll
-    for (int nu = 0; nu < ND; ++nu) own[nu] = Uin[nu * V + s];
-    const T ep = T(c.ep), eu = T(c.eu), coef = T(c.coef);
-    const T rn = T(c.round_n);
-#pragma unroll
-    for (int mu = 0; mu < ND; ++mu) {
-      acc K = G::zero(), Kc = G::zero();
-#pragma unroll
-      for (int nu = 0; nu < ND; ++nu) {
-        if (nu == mu) continue;
-        const T w = T((mu == 0 || nu == 0) ? c.w_t : c.w_s);
-        acc t1 = G::upper(Uin[nu * V + s + up[mu]], Uin[mu * V + s + up[nu]], own[nu]);
-        acc t2 = G::lower(Uin[nu * V + s + up[mu] + dn[nu]], Uin[mu * V + s + dn[nu]],
-                          Uin[nu * V + s + dn[nu]]);
-        if (c.flags & F_ANISO) {
-          t1 = G::scale(w, t1);
-          t2 = G::scale(w, t2);
-        }
-        if (c.flags & F_COMPENSATED) {
-          G::add_comp(K, Kc, t1);
-          G::add_comp(K, Kc, t2);
-        } else {
-          K = G::add(G::add(K, t1), t2);
-        }
-      }
-      if (c.flags & F_COMPENSATED) K = G::add(K, Kc);
-      T d[NM];
-      T retr;
-      G::project(own[mu], K, d, retr);
-      act += double(retr);
-      T p[NM];
-#pragma unroll
-      for (int k = 0; k < NM; ++k) {
-        p[k] = (c.flags & F_PZERO) ? T(0) : P[(mu * NM + k) * V + s];
-        kin_old += double(p[k]) * double(p[k]);
-        T f = coef * d[k];
-        if (c.flags & F_ROUND) f = ::round(f * rn) / rn;
-        p[k] -= ep * f;
-        kin_new += double(p[k]) * double(p[k]);
-        P[(mu * NM + k) * V + s] = p[k];
-      }
-      if (WRITE_U) {
-        link u = own[mu];
-        if (c.flags & F_ROUND) u = G::round(u, rn);
-        link un = G::expmul(p, eu, u);
-        if (c.flags & F_RESTORE) un = G::restore(un);
-        Uout[mu * V + s] = un;
-      }
-    }
-  }
+  if (s < geo.V) md_site<G, ND, WRITE_U>(Uin, Uout, P, geo, c, s, kin_old, act, kin_new);
   if (c.flags & F_ACCUM) {
     double r0 = block_sum<NT>(kin_old, lds);
     double r1 = block_sum<NT>(act, lds);
@@ -138,13 +81,6 @@ __global__ __launch_bounds__(NT) void k_restore(typename G::link *__restrict__ U
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i < n) U[i] = G::restore(U[i]);
 }
-
-__device__ __forceinline__ quat<double> to_d(const quat<float> &q) {
-  return quat<double>{double(q.ar), double(q.ai), double(q.br), double(q.bi)};
-}
-__device__ __forceinline__ quat<double> to_d(const quat<double> &q) { return q; }
-__device__ __forceinline__ double to_d(float x) { return double(x); }
-__device__ __forceinline__ double to_d(double x) { return x; }
 
 // Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) computed in
 // FP64 whatever the storage precision.  With F_RESTORE each link is restored on
@@ -396,7 +332,18 @@ __global__ __launch_bounds__(NT) void k_momenta_philox(T *__restrict__ P, int V,
   for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * V + s] = T(z[k]);
 }
 
+// plain streaming copy (16 B per lane per iteration): the measured HBM copy
+// bandwidth and the calibration of the FETCH_SIZE / WRITE_SIZE counters
+__global__ __launch_bounds__(NT) void k_copy16(const float4 *__restrict__ src,
+                                               float4 *__restrict__ dst, long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT)
+    dst[i] = src[i];
+}
+
 // ------------------------------------------------------------- launchers ----
+void launch_copy16(const void *src, void *dst, long n16, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(NT), 0, st, (const float4 *)src, (float4 *)dst, n16);
+}
 static inline int nblk(long n) { return (int)((n + NT - 1) / NT); }
 
 template <class G, int ND>

@@ -1,0 +1,81 @@
+// su2_md_site.hpp -- the per-site body of the fused force / momentum / link
+// update, shared by the single-lattice kernel (su2_kernels.hip) and the batched
+// ensemble kernel (su2_batch.hip).  For every link (s, mu) of site s:
+//   K  = sum_{nu != mu} w_{mu nu} [upper + lower staple]   (get_staples.hh:26-80)
+//   F  = coef * get_deriv(U_mu(s) K)                       (flat-gaugemonomial.hh:105-131)
+//   P <- P - ep * F                                        (update_momenta.hh:27-30)
+//   U' = exp(eu * P) U_mu(s) -> Uout                       (update_gauge.hh:11-14)
+#pragma once
+#include "su2_device.hpp"
+#include "su2_launch.hpp"
+
+namespace su2k {
+
+using namespace su2dev;
+
+template <class G, int ND, bool WRITE_U>
+__device__ __forceinline__ void md_site(const typename G::link *__restrict__ Uin,
+                                        typename G::link *__restrict__ Uout,
+                                        typename G::real *__restrict__ P, const Geo &geo,
+                                        const FusedCoef &c, int s, double &kin_old, double &act,
+                                        double &kin_new) {
+  using T = typename G::real;
+  using link = typename G::link;
+  using acc = typename G::acc;
+  constexpr int NM = G::NMOM;
+  const int V = geo.V;
+  int up[ND], dn[ND];
+  neighbour_offsets<ND>(geo, s, up, dn);
+  link own[ND];
+#pragma unroll
+  for (int nu = 0; nu < ND; ++nu) own[nu] = Uin[nu * V + s];
+  const T ep = T(c.ep), eu = T(c.eu), coef = T(c.coef);
+  const T rn = T(c.round_n);
+#pragma unroll
+  for (int mu = 0; mu < ND; ++mu) {
+    acc K = G::zero(), Kc = G::zero();
+#pragma unroll
+    for (int nu = 0; nu < ND; ++nu) {
+      if (nu == mu) continue;
+      const T w = T((mu == 0 || nu == 0) ? c.w_t : c.w_s);
+      acc t1 = G::upper(Uin[nu * V + s + up[mu]], Uin[mu * V + s + up[nu]], own[nu]);
+      acc t2 = G::lower(Uin[nu * V + s + up[mu] + dn[nu]], Uin[mu * V + s + dn[nu]],
+                        Uin[nu * V + s + dn[nu]]);
+      if (c.flags & F_ANISO) {
+        t1 = G::scale(w, t1);
+        t2 = G::scale(w, t2);
+      }
+      if (c.flags & F_COMPENSATED) {
+        G::add_comp(K, Kc, t1);
+        G::add_comp(K, Kc, t2);
+      } else {
+        K = G::add(G::add(K, t1), t2);
+      }
+    }
+    if (c.flags & F_COMPENSATED) K = G::add(K, Kc);
+    T d[NM];
+    T retr;
+    G::project(own[mu], K, d, retr);
+    act += double(retr);
+    T p[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      p[k] = (c.flags & F_PZERO) ? T(0) : P[(mu * NM + k) * V + s];
+      kin_old += double(p[k]) * double(p[k]);
+      T f = coef * d[k];
+      if (c.flags & F_ROUND) f = ::round(f * rn) / rn;
+      p[k] -= ep * f;
+      kin_new += double(p[k]) * double(p[k]);
+      P[(mu * NM + k) * V + s] = p[k];
+    }
+    if (WRITE_U) {
+      link u = own[mu];
+      if (c.flags & F_ROUND) u = G::round(u, rn);
+      link un = G::expmul(p, eu, u);
+      if (c.flags & F_RESTORE) un = G::restore(un);
+      Uout[mu * V + s] = un;
+    }
+  }
+}
+
+}  // namespace su2k

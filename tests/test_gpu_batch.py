@@ -1,0 +1,132 @@
+"""GPU tests of the batched ensemble (one launch per MD tick for many lattices).
+
+Every member of a batch must follow exactly the trajectory that a single-lattice
+handle (itself pinned to the oracle in test_gpu_md.py) gives for the same links
+and momenta; graph replays must reproduce the eager launches; the autotuner
+must leave the configurations untouched.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 13526463
+
+MIXED = [
+    dict(group=0, dims=(8, 1, 1, 6, 2), beta=2.0, integrator="leapfrog", nsteps=7),
+    dict(group=1, dims=(4, 5, 1, 3, 3), beta=3.5, integrator="omf2", nsteps=3),
+    dict(group=1, dims=(4, 4, 4, 4, 4), beta=2.4, integrator="leapfrog", nsteps=10),
+    dict(group=0, dims=(17, 1, 1, 19, 2), beta=4.0, integrator="omf2", nsteps=5),
+    dict(group=1, dims=(6, 1, 1, 5, 2), beta=1.5, integrator="leapfrog", nsteps=4),
+    dict(group=1, dims=(3, 4, 5, 6, 4), beta=2.3, integrator="omf2", nsteps=6),
+]
+
+
+@pytest.fixture(scope="module")
+def S():
+    import su2_amd
+    su2_amd.load_library()
+    return su2_amd
+
+
+def _starts(oracle):
+    out = []
+    for i, m in enumerate(MIXED):
+        U = oracle.hotstart(m["group"], m["dims"], SEED + i, 0.7)
+        P = oracle.initnormal(m["group"], m["dims"], SEED + 50 + i)
+        out.append((U, P))
+    return out
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("block", [64, 256])
+def test_batch_members_match_single_lattice(S, oracle, prec, block):
+    starts = _starts(oracle)
+    with S.Batch(MIXED, precision=prec) as b:
+        b.set_launch(block, 0)
+        for i, (U, P) in enumerate(starts):
+            b.set_links(i, U)
+            b.set_momenta(i, P)
+        res = b.trajectory(0, momenta=S.MOMENTA_HOST, metropolis=False)
+        got = [b.get_links(i) for i in range(len(MIXED))]
+    for i, m in enumerate(MIXED):
+        U, P = starts[i]
+        with S.Lattice(m["group"], m["dims"], m["beta"], precision=prec) as lat:
+            lat.set_links(U)
+            r = lat.md_trajectory(P, S.INTEGRATORS[m["integrator"]], m["nsteps"])
+            lat.md_finish(True)
+            want = lat.get_links()
+        tol = 1e-11 if prec == 0 else 2e-5
+        np.testing.assert_allclose(got[i], want, atol=tol, err_msg=f"member {i}")
+        assert res[i]["deltaH"] == pytest.approx(r["deltaH"], rel=1e-9 if prec == 0 else 1e-3,
+                                                 abs=1e-10 if prec == 0 else 1e-4)
+        if prec == 0:
+            Uw, _ = oracle.integrate(m["group"], m["dims"], U, P, m["beta"],
+                                     S.INTEGRATORS[m["integrator"]], m["nsteps"])
+            np.testing.assert_allclose(got[i], Uw, atol=1e-10)
+
+
+def test_batch_reject_keeps_start(S, oracle):
+    starts = _starts(oracle)
+    with S.Batch(MIXED, precision=0) as b:
+        for i, (U, P) in enumerate(starts):
+            b.set_links(i, U)
+        # rough starts give large |dH|: the members whose trajectory is
+        # rejected must keep their start links, the others must move
+        res = b.trajectory(3, momenta=S.MOMENTA_DEVICE, metropolis=True)
+        for i, r in enumerate(res):
+            now = b.get_links(i)
+            if not r["accept"]:
+                np.testing.assert_array_equal(now, starts[i][0])
+            else:
+                assert not np.array_equal(now, starts[i][0])
+
+
+def test_graph_replay_matches_eager(S, oracle):
+    starts = _starts(oracle)
+    runs = []
+    for graph in (False, True):
+        with S.Batch(MIXED, precision=1) as b:
+            b.use_graph(graph)
+            for i, (U, _) in enumerate(starts):
+                b.set_links(i, U)
+            hist = []
+            for t in range(4):
+                res = b.trajectory(100 + t, momenta=S.MOMENTA_DEVICE)
+                # eager work and synchronisation between the replays
+                hist.append(([r["deltaH"] for r in res], [r["accept"] for r in res],
+                             [b.get_links(i).copy() for i in range(len(MIXED))]))
+            runs.append(hist)
+    for (dh0, acc0, U0), (dh1, acc1, U1) in zip(*runs):
+        assert dh0 == dh1
+        assert acc0 == acc1
+        for a, c in zip(U0, U1):
+            np.testing.assert_array_equal(a, c)
+
+
+def test_autotune_cache_and_no_side_effects(S, oracle, tmp_path):
+    starts = _starts(oracle)
+    cache = str(tmp_path / "tune.json")
+    with S.Batch(MIXED, precision=1) as b:
+        for i, (U, _) in enumerate(starts):
+            b.set_links(i, U)
+        before = [b.get_links(i).copy() for i in range(len(MIXED))]
+        blk, grd = b.autotune(cache)
+        assert blk in (64, 128, 256) and grd > 0
+        for i in range(len(MIXED)):
+            np.testing.assert_array_equal(b.get_links(i), before[i])
+        assert (tmp_path / "tune.json").exists()
+        assert b.autotune(cache) == (blk, grd)
+        r = b.trajectory(7)
+        assert all(np.isfinite(x["deltaH"]) for x in r)
+
+
+def test_mixed_ensemble_runs(S):
+    """The 224-lattice mixed ensemble: one trajectory with finite dH everywhere."""
+    members = S.ensemble_members()
+    with S.Batch(members, precision=1) as b:
+        b.use_graph(True)
+        res = b.trajectory(1)
+        assert len(res) == 224
+        assert all(np.isfinite(r["deltaH"]) for r in res)
+        assert b.link_updates() > 1e8

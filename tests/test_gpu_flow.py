@@ -1,0 +1,91 @@
+"""GPU tests of the gradient-flow adjoint (forward/backward C-ABI pair).
+
+The backward pass has no reference counterpart; it is pinned by
+  * central finite differences of the forward observable (FP64, h = 1e-4) along
+    random su(2) direction fields X: (O(e^{hX}U) - O(e^{-hX}U)) / 2h = X . grad;
+  * the plain force kernel: with zero flow steps the gradient of the Wilson
+    action is gaugemonomial::derivative (flat-gaugemonomial.hh:105-131);
+  * the forward flow itself, which is checked against the oracle's RK3 steps
+    (tests/test_gpu_md.py::test_runge_kutta_and_flow_parity).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 13526463
+
+
+@pytest.fixture(scope="module")
+def S():
+    import su2_amd
+    su2_amd.load_library()
+    return su2_amd
+
+
+def _obs_at(lat, U, X, h, eps, nsteps, obs):
+    lat.set_links(U)
+    if h != 0.0:
+        lat.set_momenta(X)
+        lat.update_gauge(h)
+    return lat.flow_forward(eps, nsteps, 4, obs)
+
+
+@pytest.mark.parametrize("obs", [0, 1])
+@pytest.mark.parametrize("nsteps,ckpt", [(0, 1), (3, 1), (7, 3)])
+def test_flow_gradient_finite_differences(S, oracle, obs, nsteps, ckpt):
+    dims = (4, 4, 4, 4, 4)
+    U = oracle.random_gauge_trafo(1, dims, oracle.hotstart(1, dims, SEED, 0.6), 654321)
+    eps = 0.02
+    rng = np.random.default_rng(5)
+    with S.Lattice(S.SU2, dims, 2.4) as lat:
+        lat.set_links(U)
+        lat.flow_forward(eps, nsteps, ckpt, obs)
+        g = lat.flow_backward()
+        h = 1e-4
+        for _ in range(4):
+            # a random su(2) direction field of unit norm
+            X = rng.normal(size=g.shape)
+            X /= np.linalg.norm(X)
+            fd = (_obs_at(lat, U, X, h, eps, nsteps, obs) - _obs_at(lat, U, X, -h, eps, nsteps, obs)) / (2 * h)
+            an = float(np.sum(X * g))
+            assert an == pytest.approx(fd, rel=1e-6, abs=1e-10), (an, fd)
+
+
+def test_zero_step_action_gradient_is_the_force(S, oracle):
+    dims = (4, 4, 4, 4, 4)
+    U = oracle.random_gauge_trafo(1, dims, oracle.hotstart(1, dims, SEED), 654321)
+    with S.Lattice(S.SU2, dims, 2.4) as lat:
+        lat.set_links(U)
+        S0 = lat.flow_forward(0.01, 0, 1, S.OBS_WILSON_ACTION)
+        assert S0 == pytest.approx(lat.gauge_action(), rel=1e-14)
+        g = lat.flow_backward()
+        F = lat.derivative(1.0)
+    np.testing.assert_allclose(g, F, rtol=0, atol=1e-13 * np.abs(F).max())
+    np.testing.assert_allclose(F, oracle.derivative(1, dims, U, 2.4), atol=1e-12)
+
+
+def test_checkpointing_is_exact_recomputation(S, oracle):
+    dims = (6, 6, 6, 6, 4)
+    U = oracle.hotstart(1, dims, SEED, 0.5)
+    with S.Lattice(S.SU2, dims, 2.4, precision=S.FP32) as lat:
+        lat.set_links(U)
+        v1 = lat.flow_forward(0.01, 25, 1)
+        g1 = lat.flow_backward()
+        v2 = lat.flow_forward(0.01, 25, 20)
+        g2 = lat.flow_backward()
+    assert v1 == v2
+    np.testing.assert_array_equal(g1, g2)
+
+
+def test_fp32_gradient_close_to_fp64(S, oracle):
+    dims = (8, 8, 8, 8, 4)
+    U = oracle.hotstart(1, dims, SEED, 0.4)
+    grads = []
+    for prec in (S.FP64, S.FP32):
+        with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
+            lat.set_links(U)
+            lat.flow_forward(0.01, 40, 20)
+            grads.append(lat.flow_backward())
+    rel = np.abs(grads[1] - grads[0]).max() / np.abs(grads[0]).max()
+    assert rel < 1e-4, rel

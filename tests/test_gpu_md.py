@@ -9,6 +9,7 @@ sweep, 1e-10 after a full trajectory; dH to 1e-9 relative (both sides use
 compensated sums); plaquette and energy density to 1e-12 relative.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -291,6 +292,61 @@ def test_gauge_invariance_and_reversibility_32(S, oracle):
         D = lat.derivative(1.0)
         want = oracle.derivative(1, dims, U, 2.4)
         np.testing.assert_allclose(D, want, atol=1e-12 * np.abs(want).max())
+        # reversibility of OMF2 from the (far from equilibrium) hot start
         r = lat.md_update(SEED, 6, 20, revtest=True)
         assert abs(r["deltadeltaH"]) < 1e-7
-        assert abs(r["deltaH"]) < 5.0
+
+
+def test_host_program_matches_oracle(S, oracle, tmp_path):
+    """The C++ host program (hmc.hpp's flow over the adapter header) on a 4^4
+    lattice: output lines against the oracle's HMC program, then a restart from
+    nconf_counter.txt (base_program.hpp:284-297) continues the chain."""
+    import subprocess
+    from su2_amd import host
+    exe = host.EXE
+    assert os.path.exists(exe), "host program not built"
+    conf = tmp_path / "confs"
+    inp = tmp_path / "in.yaml"
+    inp.write_text(f"""geometry:
+  L : 4
+  T : 4
+  ndims : 4
+hmc:
+  seed: {SEED}
+  n_save: 3
+  n_meas: 6
+  heat: true
+  lenghty_conf_name: false
+  conf_dir: "{conf}/"
+integrator:
+  name: omf2
+  N_rev: 2
+  n_steps: 10
+  tau: 1.0
+monomials:
+  gauge:
+    beta: 2.4
+""")
+    out = subprocess.run([exe, "-f", str(inp)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = [ln.split() for ln in out.stdout.splitlines()
+             if ln and ln[0].isdigit() and len(ln.split()) == 7]
+    _, rows = oracle.hmc_run(1, (4, 4, 4, 4, 4), 2.4, SEED, 6, 10, 6, N_rev=2)
+    assert len(lines) == 6
+    for ln, r in zip(lines, rows):
+        assert int(ln[0]) == r[0] and int(ln[1]) == r[1]
+        assert float(ln[2]) == pytest.approx(r[2], rel=1e-12)
+        assert float(ln[3]) == pytest.approx(r[3], abs=1e-9)
+        if ln[5] != "NA":
+            assert float(ln[5]) == pytest.approx(r[5], abs=1e-9)
+        assert float(ln[6]) == pytest.approx(r[6], rel=1e-9, abs=1e-12)
+    assert (conf / "conf.3").exists() and (conf / "conf.final").exists()
+    counter = (conf / "nconf_counter.txt").read_text().splitlines()
+    assert counter[0] == "heat i path_conf" and counter[1].split()[1] == "3"
+    # restart continues at i = 3
+    inp.write_text(inp.read_text().replace("heat: true", "restart: true").replace("n_meas: 6", "n_meas: 2"))
+    out2 = subprocess.run([exe, "-f", str(inp)], capture_output=True, text=True, timeout=120)
+    assert out2.returncode == 0, out2.stderr
+    lines2 = [ln.split() for ln in out2.stdout.splitlines()
+              if ln and ln[0].isdigit() and len(ln.split()) == 7]
+    assert [int(x[0]) for x in lines2] == [3, 4]
