@@ -64,8 +64,8 @@ struct su2gpu_batch {
   size_t ubytes = 0, pelems = 0;
   su2k::BArenas ar{};
   su2k::BLat *dlat = nullptr;
-  int2 *dtiles = nullptr;
-  int ntiles = 0, block = 256, grid = 0, maxpairs = 0;
+  int2 *dtiles = nullptr, *dltiles = nullptr;
+  int ntiles = 0, nltiles = 0, block = 256, grid = 0, grid_site = 0, maxpairs = 0;
   double *partials = nullptr, *dsums = nullptr, *hsums = nullptr;
   uint64_t *dtraj = nullptr, *htraj = nullptr;
   double *staging = nullptr, *hstage = nullptr;
@@ -84,27 +84,39 @@ using B = su2gpu_batch;
 
 int npairs_of(int kind, size_t n) { return kind == SU2GPU_OMF2 ? (int)(2 * n) : (int)n; }
 
-// tile list for tile size = block; lattices contiguous
+// site tiles (tile size = block) and link tiles (one per site tile and mu, mu
+// fastest, site start packed as 4*s + mu); lattices contiguous
 int build_tiles(B *b) {
-  std::vector<int2> tiles;
+  std::vector<int2> tiles, ltiles;
   for (size_t l = 0; l < b->lat.size(); ++l) {
     auto &L = b->lat[l];
     L.tile0 = (int)tiles.size();
-    for (int s = 0; s < L.geo.V; s += b->block) tiles.push_back(int2{(int)l, s});
+    L.ltile0 = (int)ltiles.size();
+    for (int s = 0; s < L.geo.V; s += b->block) {
+      tiles.push_back(int2{(int)l, s});
+      for (int mu = 0; mu < L.geo.nd; ++mu) ltiles.push_back(int2{(int)l, 4 * s + mu});
+    }
     L.ntiles = (int)tiles.size() - L.tile0;
+    L.nltiles = (int)ltiles.size() - L.ltile0;
+    L.zs = (64 % L.geo.L[L.geo.nd - 1] == 0) && (L.geo.V % b->block == 0);
   }
   b->ntiles = (int)tiles.size();
-  if (b->dtiles) BH(hipFree(b->dtiles));
-  b->dtiles = nullptr;
+  b->nltiles = (int)ltiles.size();
+  for (int2 **p : {&b->dtiles, &b->dltiles}) {
+    if (*p) BH(hipFree(*p));
+    *p = nullptr;
+  }
   BH(hipMalloc(&b->dtiles, tiles.size() * sizeof(int2)));
   BH(hipMemcpy(b->dtiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice));
+  BH(hipMalloc(&b->dltiles, ltiles.size() * sizeof(int2)));
+  BH(hipMemcpy(b->dltiles, ltiles.data(), ltiles.size() * sizeof(int2), hipMemcpyHostToDevice));
   if (b->partials) BH(hipFree(b->partials));
   b->partials = nullptr;
-  BH(hipMalloc(&b->partials, (size_t)NSUM * b->ntiles * sizeof(double)));
-  if (b->grid <= 0 || b->grid > b->ntiles) {
-    // default: one pass over the tiles, rounded to a multiple of the 8 XCDs
-    b->grid = ((b->ntiles + 7) / 8) * 8;
-  }
+  BH(hipMalloc(&b->partials, (size_t)NSUM * b->nltiles * sizeof(double)));
+  // site kernels: one pass over the site tiles; MD ticks: `grid` blocks (default
+  // one pass over the link tiles), multiples of the 8 XCDs
+  b->grid_site = ((b->ntiles + 7) / 8) * 8;
+  if (b->grid <= 0 || b->grid > b->nltiles) b->grid = ((b->nltiles + 7) / 8) * 8;
   if (b->gexec) {
     hipGraphExecDestroy(b->gexec);
     b->gexec = nullptr;
@@ -131,20 +143,20 @@ int fin_arena(const su2k::BLat &L) {
 int enqueue(B *b, bool devmom) {
   const int n = (int)b->lat.size();
   if (devmom)
-    su2k::launch_batch_momenta(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->dtraj, b->grid,
-                               b->block, b->stream);
-  su2k::launch_batch_plaq(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, 0, 4, b->partials,
-                          b->grid, b->block, b->stream);
+    su2k::launch_batch_momenta(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->dtraj,
+                               b->grid_site, b->block, b->stream);
+  su2k::launch_batch_plaq(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, 0, 4, b->nltiles,
+                          b->partials, b->grid_site, b->block, b->stream);
   const int flags = (b->prec == 1) ? (su2k::F_COMPENSATED | su2k::F_RESTORE) : 0;
   for (int t = 0; t <= b->maxpairs; ++t)
-    su2k::launch_batch_tick(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, t, flags, b->partials,
-                            b->grid, b->block, b->stream);
+    su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->ar, t, flags,
+                            b->partials, b->grid, b->block, b->stream);
   if (b->prec == 0)
-    su2k::launch_batch_restore(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->grid, b->block,
-                               b->stream);
-  su2k::launch_batch_plaq(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, 1, 3, b->partials,
-                          b->grid, b->block, b->stream);
-  su2k::launch_batch_reduce(b->dlat, n, b->partials, b->ntiles, b->dsums, b->stream);
+    su2k::launch_batch_restore(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->grid_site,
+                               b->block, b->stream);
+  su2k::launch_batch_plaq(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, 1, 3, b->nltiles,
+                          b->partials, b->grid_site, b->block, b->stream);
+  su2k::launch_batch_reduce(b->dlat, n, b->partials, b->nltiles, b->dsums, b->stream);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(SU2GPU_EHIP, "batch launch: %s", hipGetErrorString(e));
   BH(hipMemcpyAsync(b->hsums, b->dsums, (size_t)n * 2 * NSUM * sizeof(double),
@@ -324,7 +336,7 @@ int su2gpu_batch_destroy(su2gpu_batch *b) {
   if (b->graph) hipGraphDestroy(b->graph);
   for (int k = 0; k < 3; ++k)
     if (b->ar.U[k]) hipFree(b->ar.U[k]);
-  for (void *p : {(void *)b->ar.P, (void *)b->dlat, (void *)b->dtiles, (void *)b->partials,
+  for (void *p : {(void *)b->ar.P, (void *)b->dlat, (void *)b->dtiles, (void *)b->dltiles, (void *)b->partials,
                   (void *)b->dsums, (void *)b->dtraj, (void *)b->staging})
     if (p) hipFree(p);
   for (void *p : {(void *)b->hsums, (void *)b->htraj, (void *)b->hstage})
@@ -452,7 +464,11 @@ int su2gpu_batch_autotune(su2gpu_batch *b, const char *cache_path, int *block_ou
   } else {
     // a trajectory never writes a lattice's start arena, so restoring the
     // cursors after the timing runs leaves the configurations untouched
-    const auto saved_lat = b->lat;
+    std::vector<int> saved_cur;
+    for (auto &L : b->lat) saved_cur.push_back(L.cur);
+    auto restore_cur = [&]() {
+      for (size_t l = 0; l < b->lat.size(); ++l) b->lat[l].cur = saved_cur[l];
+    };
     std::vector<su2gpu_md_result> res(b->lat.size());
     const bool g = b->use_graph;
     b->use_graph = false;
@@ -469,10 +485,9 @@ int su2gpu_batch_autotune(su2gpu_batch *b, const char *cache_path, int *block_ou
       for (int mult : {1, 2, 4, 8, 0}) {
         const int gr = mult == 0 ? full : std::min(full, ((mult * b->ncu + 7) / 8) * 8);
         b->grid = gr;
-        b->lat = saved_lat;
-        BK(upload_desc(b));
+        restore_cur();
         BK(run_trajectory(b, true, 0x7f7f7f7full, 0, res.data()));  // warm-up
-        b->lat = saved_lat;
+        restore_cur();
         BH(hipEventRecord(e0, b->stream));
         BK(run_trajectory(b, true, 0x7f7f7f7full, 0, res.data()));
         BH(hipEventRecord(e1, b->stream));
@@ -488,7 +503,7 @@ int su2gpu_batch_autotune(su2gpu_batch *b, const char *cache_path, int *block_ou
     }
     BH(hipEventDestroy(e0));
     BH(hipEventDestroy(e1));
-    b->lat = saved_lat;
+    restore_cur();
     b->use_graph = g;
     b->block = bb;
     b->grid = bg;

@@ -34,25 +34,27 @@ __device__ __forceinline__ double final_coef(int kind) { return kind == 6 ? 0.19
 
 template <typename T, class G, int ND, bool WRITE_U>
 __device__ __forceinline__ void tile_step(const BLat &L, char *const *arena, T *Parena, int rd, int wr,
-                                          const FusedCoef &c, int s, double &k0, double &a, double &k1) {
+                                          const FusedCoef &c, int s, int mu, double &k0, double &a,
+                                          double &k1) {
   using link = typename G::link;
   const link *Uin = (const link *)(arena[rd] + L.uoff);
   link *Uout = WRITE_U ? (link *)(arena[wr] + L.uoff) : nullptr;
-  md_site<G, ND, WRITE_U>(Uin, Uout, Parena + L.poff, L.geo, c, s, k0, a, k1);
+  if (L.zs) md_link<G, ND, WRITE_U, true>(Uin, Uout, Parena + L.poff, L.geo, c, s, mu, k0, a, k1);
+  else md_link<G, ND, WRITE_U, false>(Uin, Uout, Parena + L.poff, L.geo, c, s, mu, k0, a, k1);
 }
 
 template <typename T, bool WRITE_U>
 __device__ __forceinline__ void dispatch_step(const BLat &L, char *const *arena, T *Parena, int rd,
-                                              int wr, const FusedCoef &c, int s, double &k0,
-                                              double &a, double &k1) {
+                                              int wr, const FusedCoef &c, int s, int mu,
+                                              double &k0, double &a, double &k1) {
   const int code = L.group * 8 + L.geo.nd;
   switch (code) {
-    case 0 * 8 + 2: tile_step<T, GU1<T>, 2, WRITE_U>(L, arena, Parena, rd, wr, c, s, k0, a, k1); break;
-    case 0 * 8 + 3: tile_step<T, GU1<T>, 3, WRITE_U>(L, arena, Parena, rd, wr, c, s, k0, a, k1); break;
-    case 0 * 8 + 4: tile_step<T, GU1<T>, 4, WRITE_U>(L, arena, Parena, rd, wr, c, s, k0, a, k1); break;
-    case 1 * 8 + 2: tile_step<T, GSU2<T>, 2, WRITE_U>(L, arena, Parena, rd, wr, c, s, k0, a, k1); break;
-    case 1 * 8 + 3: tile_step<T, GSU2<T>, 3, WRITE_U>(L, arena, Parena, rd, wr, c, s, k0, a, k1); break;
-    default: tile_step<T, GSU2<T>, 4, WRITE_U>(L, arena, Parena, rd, wr, c, s, k0, a, k1); break;
+    case 0 * 8 + 2: tile_step<T, GU1<T>, 2, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
+    case 0 * 8 + 3: tile_step<T, GU1<T>, 3, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
+    case 0 * 8 + 4: tile_step<T, GU1<T>, 4, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
+    case 1 * 8 + 2: tile_step<T, GSU2<T>, 2, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
+    case 1 * 8 + 3: tile_step<T, GSU2<T>, 3, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
+    default: tile_step<T, GSU2<T>, 4, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
   }
 }
 
@@ -69,7 +71,7 @@ __global__ __launch_bounds__(256) void k_batch_tick(const BLat *__restrict__ lat
     const int2 tl = tiles[lt];
     const BLat &L = lats[tl.x];
     if (tick > L.npairs) continue;  // lattice finished (uniform over the block)
-    const int s = tl.y + threadIdx.x;
+    const int s = (tl.y >> 2) + threadIdx.x, mu = tl.y & 3;
     const int a0 = L.cur, a1 = (L.cur + 1) % 3, a2 = (L.cur + 2) % 3;
     FusedCoef c{};
     c.coef = L.coef;
@@ -91,9 +93,9 @@ __global__ __launch_bounds__(256) void k_batch_tick(const BLat *__restrict__ lat
       rd = L.npairs == 0 ? a0 : (((L.npairs - 1) & 1) ? a2 : a1);
       wr = rd;
     }
-    if (s < L.geo.V) {
-      if (!final_p) dispatch_step<T, true>(L, ar.U, (T *)ar.P, rd, wr, c, s, k0, act, k1);
-      else dispatch_step<T, false>(L, ar.U, (T *)ar.P, rd, wr, c, s, k0, act, k1);
+    if (s < L.geo.V) {  // always true on z-shuffle lattices (V % tile == 0)
+      if (!final_p) dispatch_step<T, true>(L, ar.U, (T *)ar.P, rd, wr, c, s, mu, k0, act, k1);
+      else dispatch_step<T, false>(L, ar.U, (T *)ar.P, rd, wr, c, s, mu, k0, act, k1);
     }
     if (tick == 0 || final_p) {
       // uniform over the block: every thread of the tile takes this branch
@@ -158,7 +160,7 @@ __device__ __forceinline__ double plaq_site(const BLat &L, const char *base, int
 template <typename T>
 __global__ __launch_bounds__(256) void k_batch_plaq(const BLat *__restrict__ lats,
                                                     const int2 *__restrict__ tiles, int ntiles,
-                                                    BArenas ar, int use_final, int row,
+                                                    BArenas ar, int use_final, int row, int rowlen,
                                                     double *__restrict__ partials) {
   __shared__ double l1[4];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(256) void k_batch_plaq(const BLat *__restrict__ lat
     if (threadIdx.x == 0) {
       double r = 0.0;
       for (int i = 0; i < (int)blockDim.x / 64; ++i) r += l1[i];
-      partials[(size_t)row * ntiles + lt] = r;
+      partials[(size_t)row * rowlen + lt] = r;
     }
   }
 }
@@ -231,17 +233,19 @@ __global__ __launch_bounds__(256) void k_batch_momenta(const BLat *__restrict__ 
   }
 }
 
-// per-lattice reduction of the 5 tile rows; out[l*10 + 2*r + {0,1}] = hi, lo
+// per-lattice reduction of the 5 tile rows (rows 0-2 over link tiles, 3-4 over
+// site tiles); out[l*10 + 2*r + {0,1}] = hi, lo
 __global__ __launch_bounds__(256) void k_batch_reduce(const BLat *__restrict__ lats,
                                                       const double *__restrict__ partials,
-                                                      int ntiles, double *__restrict__ out) {
+                                                      int rowlen, double *__restrict__ out) {
   __shared__ double hs[256], ls[256];
   const BLat &L = lats[blockIdx.x];
   for (int r = 0; r < 5; ++r) {
+    const int t0 = r < 3 ? L.ltile0 : L.tile0, nt = r < 3 ? L.nltiles : L.ntiles;
     double hi = 0.0, lo = 0.0;
-    for (int i = threadIdx.x; i < L.ntiles; i += blockDim.x) {
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
       double e;
-      two_sum(hi, partials[(size_t)r * ntiles + L.tile0 + i], hi, e);
+      two_sum(hi, partials[(size_t)r * rowlen + t0 + i], hi, e);
       lo += e;
     }
     hs[threadIdx.x] = hi;
@@ -273,11 +277,12 @@ void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles
     hipLaunchKernelGGL((k_batch_tick<float>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, tick, flags, partials);
 }
 void launch_batch_plaq(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
-                       int use_final, int row, double *partials, int grid, int block, hipStream_t st) {
+                       int use_final, int row, int rowlen, double *partials, int grid, int block,
+                       hipStream_t st) {
   if (prec == 0)
-    hipLaunchKernelGGL((k_batch_plaq<double>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, use_final, row, partials);
+    hipLaunchKernelGGL((k_batch_plaq<double>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, use_final, row, rowlen, partials);
   else
-    hipLaunchKernelGGL((k_batch_plaq<float>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, use_final, row, partials);
+    hipLaunchKernelGGL((k_batch_plaq<float>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, use_final, row, rowlen, partials);
 }
 void launch_batch_restore(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
                           int grid, int block, hipStream_t st) {
@@ -293,9 +298,9 @@ void launch_batch_momenta(int prec, const BLat *lats, const int2 *tiles, int nti
   else
     hipLaunchKernelGGL((k_batch_momenta<float>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, traj);
 }
-void launch_batch_reduce(const BLat *lats, int nlat, const double *partials, int ntiles, double *out,
+void launch_batch_reduce(const BLat *lats, int nlat, const double *partials, int rowlen, double *out,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_reduce, dim3(nlat), dim3(256), 0, st, lats, partials, ntiles, out);
+  hipLaunchKernelGGL(k_batch_reduce, dim3(nlat), dim3(256), 0, st, lats, partials, rowlen, out);
 }
 
 }  // namespace su2k

@@ -15,7 +15,9 @@ struct BLat {
   int npairs;     // number of "P then U" pairs; tick npairs is the final momentum step
   int cur;        // arena holding the configuration at the start of the trajectory
   int fin;        // arena holding the last link update
-  int tile0, ntiles;
+  int tile0, ntiles;    // site tiles (sums, momenta, restore)
+  int ltile0, nltiles;  // link tiles (MD ticks): one per (site tile, mu)
+  int zs;               // z-shuffle path allowed (64 % L_fastest == 0, V % tile == 0)
   long uoff;      // byte offset of the lattice's links in every link arena
   long poff;      // element offset of its momenta in the momentum arena
   double coef;    // beta / Nc
@@ -28,16 +30,18 @@ struct BArenas {
   void *P;
 };
 
-void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
+// ltiles: (lattice, site_start * 4 + mu); partial rows are nltiles long
+void launch_batch_tick(int prec, const BLat *lats, const int2 *ltiles, int nltiles, const BArenas &ar,
                        int tick, int flags, double *partials, int grid, int block, hipStream_t st);
 void launch_batch_plaq(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
-                       int use_final, int row, double *partials, int grid, int block, hipStream_t st);
+                       int use_final, int row, int rowlen, double *partials, int grid, int block,
+                       hipStream_t st);
 void launch_batch_restore(int prec, const BLat *lats, const int2 *tiles, int ntiles,
                           const BArenas &ar, int grid, int block, hipStream_t st);
 void launch_batch_momenta(int prec, const BLat *lats, const int2 *tiles, int ntiles,
                           const BArenas &ar, const uint64_t *traj, int grid, int block,
                           hipStream_t st);
-void launch_batch_reduce(const BLat *lats, int nlat, const double *partials, int ntiles,
+void launch_batch_reduce(const BLat *lats, int nlat, const double *partials, int rowlen,
                          double *out, hipStream_t st);
 
 }  // namespace su2k

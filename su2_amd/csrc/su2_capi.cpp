@@ -271,7 +271,7 @@ int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, i
                       w_t, w_s, mu_start, flags, row(h, r), h->stream);
   }
   CK(last_launch("plaq"));
-  return reduce_row(h, r, su2k::fused_blocks(h->geo));
+  return reduce_row(h, r, su2k::site_blocks(h->geo));
 }
 
 int sumsq_launch(L *h, const void *P, int r) {
@@ -1067,7 +1067,7 @@ int su2gpu_md_update(su2gpu_lattice *h, uint64_t seed, int integrator, size_t ns
 int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, double *Q) {
   CK(check(h));
   if (!E || !Q) return fail(SU2GPU_EINVAL, "null pointer");
-  const int nbf = su2k::fused_blocks(h->geo);
+  const int nbf = su2k::site_blocks(h->geo);
   {
     Prof pr(h, "energy_density");
     su2k::launch_energy_density(h->group, h->prec, h->U[h->cur], h->geo, clover ? 1 : 0,

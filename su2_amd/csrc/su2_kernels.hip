@@ -10,6 +10,9 @@
 //       U(1) link = T phase.
 //   P : nd*NMOM planes of V reals, component c of link (s,mu) at P[(mu*NMOM+c)*V+s].
 // Sites s follow the reference's lexicographic order (gaugeconfig.hh:211-222).
+#include <cstdlib>
+#include <cstring>
+
 #include "su2_device.hpp"
 #include "su2_launch.hpp"
 #include "su2_md_site.hpp"
@@ -42,6 +45,33 @@ __global__ __launch_bounds__(NT) void k_md_fused(const typename G::link *__restr
   const int s = lb * NT + threadIdx.x;
   double kin_old = 0.0, act = 0.0, kin_new = 0.0;
   if (s < geo.V) md_site<G, ND, WRITE_U>(Uin, Uout, P, geo, c, s, kin_old, act, kin_new);
+  if (c.flags & F_ACCUM) {
+    double r0 = block_sum<NT>(kin_old, lds);
+    double r1 = block_sum<NT>(act, lds);
+    double r2 = block_sum<NT>(kin_new, lds);
+    if (threadIdx.x == 0) {
+      partials[0 * pstride + lb] = r0;
+      partials[1 * pstride + lb] = r1;
+      partials[2 * pstride + lb] = r2;
+    }
+  }
+}
+
+// The same step with one thread per link: logical block lb -> (site block, mu)
+// with mu fastest, so the nd blocks of one site range run together on one XCD.
+template <class G, int ND, bool WRITE_U, bool ZS>
+__global__ __launch_bounds__(NT) void k_md_link(const typename G::link *__restrict__ Uin,
+                                                typename G::link *__restrict__ Uout,
+                                                typename G::real *__restrict__ P, Geo geo,
+                                                FusedCoef c, double *__restrict__ partials,
+                                                int pstride) {
+  __shared__ double lds[NT / 64];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int mu = lb % ND, s = (lb / ND) * NT + threadIdx.x;
+  double kin_old = 0.0, act = 0.0, kin_new = 0.0;
+  // with ZS every lane is active (V % NT == 0 is checked by the launcher)
+  if (ZS || s < geo.V)
+    md_link<G, ND, WRITE_U, ZS>(Uin, Uout, P, geo, c, s, mu, kin_old, act, kin_new);
   if (c.flags & F_ACCUM) {
     double r0 = block_sum<NT>(kin_old, lds);
     double r1 = block_sum<NT>(act, lds);
@@ -346,10 +376,44 @@ void launch_copy16(const void *src, void *dst, long n16, hipStream_t st) {
 }
 static inline int nblk(long n) { return (int)((n + NT - 1) / NT); }
 
+// kernel shape of the fused step: SU2GPU_FUSED=site|link|linkz (default: linkz
+// where 64 % L_fastest == 0 and V % 256 == 0, else link)
+static int fused_mode_env() {
+  static int mode = -1;
+  if (mode < 0) {
+    const char *e = getenv("SU2GPU_FUSED");
+    mode = !e ? 3 : (!strcmp(e, "site") ? 0 : (!strcmp(e, "link") ? 1 : (!strcmp(e, "linkz") ? 2 : 3)));
+  }
+  return mode;
+}
+int fused_mode(const Geo &g) {
+  const int m = fused_mode_env();
+  const bool zs_ok = (64 % g.L[g.nd - 1] == 0) && (g.V % NT == 0);
+  if (m == 0) return 0;
+  if (m == 1) return 1;
+  return zs_ok ? 2 : 1;
+}
+
 template <class G, int ND>
 static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
   using link = typename G::link;
   using T = typename G::real;
+  const int mode = fused_mode(L.geo);
+  if (mode != 0) {
+    dim3 grid(ND * nblk(L.geo.V)), block(NT);
+#define SU2_LINK(W, Z)                                                                     \
+  hipLaunchKernelGGL((k_md_link<G, ND, W, Z>), grid, block, 0, st, (const link *)L.Uin,   \
+                     (link *)L.Uout, (T *)L.P, L.geo, L.c, L.partials, L.pstride)
+    if (L.Uout) {
+      if (mode == 2) SU2_LINK(true, true);
+      else SU2_LINK(true, false);
+    } else {
+      if (mode == 2) SU2_LINK(false, true);
+      else SU2_LINK(false, false);
+    }
+#undef SU2_LINK
+    return;
+  }
   dim3 grid(nblk(L.geo.V)), block(NT);
   if (L.Uout)
     hipLaunchKernelGGL((k_md_fused<G, ND, true>), grid, block, 0, st, (const link *)L.Uin,
@@ -377,7 +441,8 @@ void launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st) {
   }
 }
 
-int fused_blocks(const Geo &g) { return nblk(g.V); }
+int fused_blocks(const Geo &g) { return fused_mode(g) == 0 ? nblk(g.V) : g.nd * nblk(g.V); }
+int site_blocks(const Geo &g) { return nblk(g.V); }
 
 template <class G>
 static void launch_update_gauge_g(void *U, const void *P, int V, int nd, double eps,

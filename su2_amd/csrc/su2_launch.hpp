@@ -45,7 +45,9 @@ struct FusedLaunch {
 };
 
 void launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st);
-int fused_blocks(const Geo &g);
+int fused_blocks(const Geo &g);  // blocks (= partial-sum entries) of one fused launch
+int site_blocks(const Geo &g);   // blocks of the one-thread-per-site sum kernels
+int fused_mode(const Geo &g);    // 0 thread per site, 1 thread per link, 2 link + z shuffles
 void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int nd, double eps,
                          double round_n, int flags, hipStream_t st);
 void launch_restore(int group, int prec, void *U, int n, hipStream_t st);
