@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstddef>
+#include <fstream>
 #include <iostream>
 #include <list>
 #include <random>
@@ -267,6 +268,26 @@ template <class G>
 void energy_density(const gaugeconfig<G> &U, double &res, double &Q, bool cloverdef = true,
                     const bool &ss = false) {
   check(su2gpu_energy_density(U.handle(), cloverdef, ss, &res, &Q), "energy_density");
+}
+// gradient_flow (flat-gradient_flow.hh:70-197): flows a copy of U on the GPU and
+// writes the reference's table (header when tstart == 0, rows in scientific
+// notation with 16 digits) to `path`, appending when tstart != 0
+template <class G>
+void gradient_flow(const gaugeconfig<G> &U, std::string const &path, const double &tmax,
+                   const double &eps, const double & /*xi: the lattice's own*/,
+                   const double &tstart, const bool & /*save_config*/) {
+  std::vector<double> rows((size_t)(tmax / eps + 4) * 9);
+  size_t n = 0;
+  check(su2gpu_gradient_flow(U.handle(), tmax, eps, tstart, rows.data(), rows.size() / 9, &n),
+        "gradient_flow");
+  std::ofstream ofs(path, tstart == 0.0 ? std::ios::out : std::ios::app);
+  if (tstart == 0.0) ofs << "t xi P P_ss Ep Ep_ss Ec Ec_ss Q" << std::endl;
+  ofs << std::scientific;
+  ofs.precision(16);
+  for (size_t k = 0; k < n; ++k) {
+    const double *r = &rows[9 * k];
+    for (int c = 0; c < 9; ++c) ofs << r[c] << (c < 8 ? " " : "\n");
+  }
 }
 }  // namespace flat_spacetime
 

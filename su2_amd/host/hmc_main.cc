@@ -118,6 +118,15 @@ template <class G> int run(const Map &in) {
   const double tau = get<double>(in, "integrator.tau", 1.0);
   const size_t exponent = get<size_t>(in, "integrator.exponent", 0);
   const int prec = gets(in, "gpu.precision", "fp64") == "fp32" ? SU2GPU_FP32 : SU2GPU_FP64;
+  // online gradient-flow measurement (omeasurements.hpp:65-91 via base_program.hpp:408-413)
+  const bool do_omeas = getb(in, "hmc.do_omeas", false) || in.count("omeas.gradient_flow.tmax");
+  const std::string res_dir = gets(in, "omeas.res_dir", ".");
+  const size_t om_icounter = get<size_t>(in, "omeas.icounter", 0);
+  const size_t om_nstep = get<size_t>(in, "omeas.nstep", 1);
+  const size_t om_nmeas = get<size_t>(in, "omeas.n_meas", 1000000000);
+  const double gf_eps = get<double>(in, "omeas.gradient_flow.epsilon", 0.01);
+  const double gf_tmax = get<double>(in, "omeas.gradient_flow.tmax", 0.0);
+  if (do_omeas && gf_tmax > 0) mkdir(res_dir.c_str(), 0755);
 
   std::cout << "## HMC Algorithm on MI355X (gfx950)\n";
   mkdir(conf_dir.c_str(), 0755);
@@ -202,6 +211,16 @@ template <class G> int run(const Map &in) {
     if (i > 0 && (i % N_save) == 0) {
       const std::string path_i = conf_path_basename + "." + std::to_string(i);
       U.save(path_i);
+      // hmc.hpp:253-261 and base_program.hpp:367-413: flow the accepted
+      // configuration, table in res_dir/gradient_flow.%06d
+      const bool om = do_omeas && gf_tmax > 0 && i > om_icounter &&
+                      (i - om_icounter) <= om_nmeas && (i % om_nstep) == 0 &&
+                      mdparams.getaccept();
+      if (om) {
+        std::ostringstream gp;
+        gp << res_dir << "/gradient_flow." << std::setw(6) << std::setfill('0') << i;
+        S::flat_spacetime::gradient_flow(U, gp.str(), gf_tmax, gf_eps, xi, 0.0, false);
+      }
       std::ofstream nc(conf_dir + "nconf_counter.txt", std::ios::out);
       nc << "heat i path_conf\n" << double(g_heat) << " " << i << " " << path_i;
     }

@@ -326,6 +326,12 @@ integrator:
 monomials:
   gauge:
     beta: 2.4
+omeas:
+  res_dir: "{tmp_path / 'omeas'}"
+  nstep: 3
+  gradient_flow:
+    epsilon: 0.02
+    tmax: 0.1
 """)
     out = subprocess.run([exe, "-f", str(inp)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
@@ -341,6 +347,17 @@ monomials:
             assert float(ln[5]) == pytest.approx(r[5], abs=1e-9)
         assert float(ln[6]) == pytest.approx(r[6], rel=1e-9, abs=1e-12)
     assert (conf / "conf.3").exists() and (conf / "conf.final").exists()
+    # online gradient flow of trajectory 3 (if it was accepted) against the oracle
+    gf = tmp_path / "omeas" / "gradient_flow.000003"
+    if rows[3][1]:
+        lines_gf = gf.read_text().splitlines()
+        assert lines_gf[0] == "t xi P P_ss Ep Ep_ss Ec Ec_ss Q"
+        table = np.array([[float(v) for v in ln.split()] for ln in lines_gf[1:]])
+        raw = np.frombuffer((conf / "conf.3").read_bytes(), dtype=np.float64).reshape(-1, 5)[:, 1:]
+        _, want = oracle.gradient_flow(1, (4, 4, 4, 4, 4), raw.copy(), 2.4, 0.1, 0.02)
+        np.testing.assert_allclose(table, want, rtol=1e-10, atol=1e-12)
+    else:
+        assert not gf.exists()
     counter = (conf / "nconf_counter.txt").read_text().splitlines()
     assert counter[0] == "heat i path_conf" and counter[1].split()[1] == "3"
     # restart continues at i = 3
