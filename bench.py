@@ -84,33 +84,68 @@ def run_form(S, prec, L, nsteps, steps, warmup, seed, beta):
     }
 
 
-def delta_dH(S, L, nsteps, beta, seed, ntraj, ntherm=30):
-    """FP32 vs FP64 from identical (thermalised) start and momenta, and the
-    reversibility violations of both forms."""
+def delta_dH(S, L, nsteps, beta, seed, ntraj, ntherm=50, nchain=40):
+    """FP32 form against the FP64 form at equilibrium.
+
+    Start: Haar hot start + `ntherm` FP64 OMF2 trajectories (the relaxation
+    trajectories lower H and are accepted).  Paired trajectories: from the same
+    links and the same Philox momentum stream, both forms run one trajectory
+    with the reversibility test; dH32 - dH64, both forms' ddH (|H after
+    forward + flipped backward - H_start|) and the plaquette after the
+    trajectory are compared.  Chains: both forms then continue `nchain`
+    trajectories from the same start; acceptance and mean plaquette are
+    compared with their binomial / standard errors."""
+    import statistics
     dims = (L, L, L, L, 4)
-    out = []
+    norm = 1.0 / (6 * L ** 4 * 2)
     l64 = S.Lattice(S.SU2, dims, beta, precision=S.FP64)
     l32 = S.Lattice(S.SU2, dims, beta, precision=S.FP32)
-    l64.coldstart()
-    # thermalise in FP64 (cold start, plaquette relaxes within a few trajectories)
+    l64.haarstart(seed)
     for i in range(ntherm):
         l64.md_update(seed + 100 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+    start = l64.get_links()
+    plaq_start = l64.gauge_energy() * norm
+    pairs = []
     for i in range(ntraj):
-        l32.set_links(l64.get_links())
+        x = l64.get_links()
+        l32.set_links(x)
         r64 = l64.md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE, revtest=True)
         r32 = l32.md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE, revtest=True)
-        out.append((r32["deltaH"] - r64["deltaH"], r32["deltadeltaH"], r64["deltadeltaH"],
-                    r32["accept"] == r64["accept"]))
-    plaq = l64.gauge_energy() / (6 * L ** 4 * 2)
+        # plaquette of each form's proposal from S_new = -(beta / 2) sum ReTr
+        pairs.append({"dH64": r64["deltaH"], "dH32": r32["deltaH"],
+                      "ddH64": r64["deltadeltaH"], "ddH32": r32["deltadeltaH"],
+                      "same_accept": r32["accept"] == r64["accept"],
+                      "plaq64": -r64["gauge_new"] / (beta / 2) * norm,
+                      "plaq32": -r32["gauge_new"] / (beta / 2) * norm})
+    # independent chains from the same equilibrium start
+    l64.set_links(start)
+    l32.set_links(start)
+    chain = {}
+    for name, lat in (("fp64", l64), ("fp32", l32)):
+        acc, pl = [], []
+        for i in range(nchain):
+            r = lat.md_update(seed + 300 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+            acc.append(r["accept"])
+            pl.append(lat.gauge_energy() * norm)
+        a = sum(acc) / nchain
+        chain[name] = {"acceptance": a, "acceptance_err": math.sqrt(max(a * (1 - a), 1e-12) / nchain),
+                       "mean_plaquette": statistics.fmean(pl),
+                       "plaquette_err": statistics.stdev(pl) / math.sqrt(nchain)}
     l64.close()
     l32.close()
+    d = [abs(p["dH32"] - p["dH64"]) for p in pairs]
     return {
-        "ntraj": ntraj,
-        "start": f"cold start + {ntherm} FP64 OMF2 trajectories, plaquette {plaq:.5f}",
-        "max_abs_dH32_minus_dH64": max(abs(o[0]) for o in out),
-        "max_abs_ddH_fp32": max(abs(o[1]) for o in out),
-        "max_abs_ddH_fp64": max(abs(o[2]) for o in out),
-        "same_accept_decisions": sum(o[3] for o in out),
+        "lattice": [L] * 4, "beta": beta,
+        "start": f"Haar hot start + {ntherm} FP64 OMF2({nsteps}) trajectories, plaquette {plaq_start:.5f}",
+        "paired_trajectories": ntraj,
+        "mean_abs_dH64": statistics.fmean(abs(p["dH64"]) for p in pairs),
+        "max_abs_dH32_minus_dH64": max(d),
+        "mean_abs_dH32_minus_dH64": statistics.fmean(d),
+        "max_abs_ddH_fp32": max(abs(p["ddH32"]) for p in pairs),
+        "max_abs_ddH_fp64": max(abs(p["ddH64"]) for p in pairs),
+        "max_abs_plaquette_diff": max(abs(p["plaq32"] - p["plaq64"]) for p in pairs),
+        "same_accept_decisions": sum(p["same_accept"] for p in pairs),
+        "chains": dict(chain, trajectories=nchain),
     }
 
 
@@ -141,6 +176,40 @@ def batch_ensemble(S, steps, warmup):
             "autotune_s": round(t_tune, 3), "graph": True, "precision": "fp32"}
 
 
+def flow_adjoint(S, L=24, beta=2.4, eps=0.01, tmax=2.0, ckpt=20, ntherm=50, seed=4242):
+    """Forward RK3 Wilson flow to tmax and the hand-written backward pass giving
+    d(t^2 <E(t)>)/dU, FP32 with FP64 sums, checkpoints every `ckpt` steps; start
+    from `ntherm` FP64 OMF2 trajectories after a Haar hot start."""
+    dims = (L, L, L, L, 4)
+    lat64 = S.Lattice(S.SU2, dims, beta, precision=S.FP64)
+    lat64.haarstart(seed)
+    for i in range(ntherm):
+        lat64.md_update(seed + i, S.OMF2, 10, momenta=S.MOMENTA_DEVICE)
+    U = lat64.get_links()
+    lat64.close()
+    lat = S.Lattice(S.SU2, dims, beta, precision=S.FP32)
+    lat.set_links(U)
+    nsteps = int(round(tmax / eps))
+    lat.flow_forward(eps, nsteps, ckpt)  # warm-up (allocations, code objects)
+    lat.flow_backward()
+    lat.set_links(U)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    val = lat.flow_forward(eps, nsteps, ckpt)
+    lat.synchronize()
+    t1 = time.perf_counter()
+    g = lat.flow_backward()
+    lat.synchronize()
+    t2 = time.perf_counter()
+    lat.close()
+    V = L ** 4
+    return {"lattice": [L] * 4, "eps": eps, "tmax": tmax, "steps": nsteps, "checkpoint_every": ckpt,
+            "precision": "fp32", "t2E": val, "forward_ms": 1000 * (t1 - t0),
+            "backward_ms": 1000 * (t2 - t1),
+            "backward_link_steps_per_s": 4 * V * nsteps / (t2 - t1),
+            "grad_norm": float((g ** 2).sum()) ** 0.5}
+
+
 def cpu_baseline(L, nsteps, beta, seed):
     """The reference's own md_update (oracle/_ref build of its hot-path headers),
     one OMF2 trajectory at the benchmark size on all host cores; falls back to
@@ -168,6 +237,21 @@ def cpu_baseline(L, nsteps, beta, seed):
                       f"{dt:.1f} s wall"}
 
 
+def measured_traffic(L, form):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor that
+    the same counter reads for a copy of known size); None if no summary for
+    this lattice size and form is committed."""
+    path = os.path.join(ROOT, "profiles", "k_md_link_traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = tab.get(f"{L}^4/{form}")
+    return None if rec is None else rec.get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,8 +262,12 @@ def main():
     ap.add_argument("--beta", type=float, default=2.4)
     ap.add_argument("--seed", type=int, default=13526463)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--ddh-traj", type=int, default=3)
+    ap.add_argument("--ddh-traj", type=int, default=8, help="paired FP32/FP64 trajectories")
+    ap.add_argument("--chain", type=int, default=40, help="chain length of the acceptance comparison")
+    ap.add_argument("--prec-L", type=int, default=48, help="lattice extent of the FP32-vs-FP64 study")
+    ap.add_argument("--prec-beta", type=float, default=2.5, help="beta of the FP32-vs-FP64 study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-flow", action="store_true", help="skip the flow-adjoint leg")
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -191,19 +279,17 @@ def main():
     S.load_library()
     f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
     f64 = run_form(S, S.FP64, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
-    ddh = delta_dH(S, args.L, args.nsteps, args.beta, args.seed, args.ddh_traj)
+    ddh = delta_dH(S, args.prec_L, args.nsteps, args.prec_beta, args.seed, args.ddh_traj,
+                   nchain=args.chain)
     copy_ms = S.copy_probe(1 << 30, 20)
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
+    flow = None if args.no_flow else flow_adjoint(S)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
-    V = args.L ** 4
-    # the compulsory-bytes definition of the metric: per link per trajectory
-    # (2n+1) force sweeps F and 2n link sweeps G (FP32: F = 40 B, G = 44 B)
     n = args.nsteps
-    traj_bytes32 = ((2 * n + 1) * 40 + 2 * n * 44) * 4 * V
-    traj_bytes64 = ((2 * n + 1) * 80 + 2 * n * 88) * 4 * V
+    kname = "k_md_link<GSU2<float>,4,true,*>"
     out = {
         "metric": "SU(2) 32^4 HMC link-updates/s, %HBM roofline; FP32 vs FP64 form at |dDH|<=1e-6",
         "value": f32["link_updates_per_s"],
@@ -222,19 +308,19 @@ def main():
                    "lattice": [args.L] * 4, "integrator": "omf2", "n_steps": n, "tau": 1.0},
         "roofline": {"bound": "hbm", "achieved": f32["fused_achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": f32["fused_achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "kernel": "k_md_fused<GSU2<float>,4,true>",
+                     "traffic": measured_traffic(args.L, "fp32"),
+                     "kernel": kname,
                      "avg_us": f32["fused_avg_us"], "bytes_per_launch": f32["fused_bytes_per_launch"]},
         "cpu_baseline": cpu,
-        "trajectory_roofline_frac_fp32": traj_bytes32 / (f32["ms_per_traj"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "fp32": f32,
-        "fp64": dict(f64, trajectory_roofline_frac=traj_bytes64 / (f64["ms_per_traj"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     fused_roofline_frac=f64["fused_achieved_GBs"] / HBM_PEAK_GBS),
+        "fp64": dict(f64, fused_roofline_frac=f64["fused_achieved_GBs"] / HBM_PEAK_GBS,
+                     traffic=measured_traffic(args.L, "fp64")),
         "fp32_over_fp64": f32["link_updates_per_s"] / f64["link_updates_per_s"],
         "delta_dH": ddh,
         "measured_copy_GBs": copy_gbs,
         "fused_frac_of_measured_copy_fp32": f32["fused_achieved_GBs"] / copy_gbs,
         "batch": batch,
+        "flow_adjoint": flow,
     }
     print(json.dumps(out), flush=True)
 

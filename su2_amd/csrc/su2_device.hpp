@@ -64,6 +64,17 @@ template <typename T> __device__ __forceinline__ quat<T> qrestore(const quat<T> 
   const T r = dsqrt(u.ar * u.ar + u.ai * u.ai + u.br * u.br + u.bi * u.bi);
   return quat<T>{u.ar / r, u.ai / r, u.br / r, u.bi / r};
 }
+// FP32 links are normalised in FP64 and rounded once per component.  An FP32
+// |q| rounds to 1 for every true |q| in (1, 1 + 2^-24), where the FP32 grid
+// is twice as coarse as below 1, so q / |q| in FP32 leaves |q|^2 - 1 biased
+// by about +3e-8 per link; over the 4 V links of a 32^4 lattice that shifts the
+// Wilson action by O(1) between a start configuration and a reunitarised one.
+// Rounding the FP64-normalised components is unbiased (mean |q|^2 - 1 ~ 1e-11).
+template <> __device__ __forceinline__ quat<float> qrestore(const quat<float> &u) {
+  const double a = u.ar, b = u.ai, c = u.br, d = u.bi;
+  const double s = 1.0 / sqrt(a * a + b * b + c * c + d * d);
+  return quat<float>{float(a * s), float(b * s), float(c * s), float(d * s)};
+}
 
 // promotion of stored links to FP64 for the FP64 sums of the FP32 form
 __device__ __forceinline__ quat<double> to_d(const quat<float> &q) {

@@ -16,6 +16,9 @@
 #include "su2_device.hpp"
 #include "su2_launch.hpp"
 #include "su2_md_site.hpp"
+#include "su2_md_fast.hpp"
+
+#include <type_traits>
 
 using namespace su2dev;
 
@@ -183,20 +186,36 @@ template <typename T> __global__ __launch_bounds__(NT) void k_scale(T *__restric
 // Deterministic final reduction of nval rows of nb block partials:
 // TwoSum-compensated strided accumulation, then a fixed-order LDS tree.
 // out[2*k] + out[2*k+1] is the sum of row k.
-__global__ __launch_bounds__(NT) void k_reduce(const double *__restrict__ partials, int nb,
-                                               double *__restrict__ out) {
-  __shared__ double hs[NT], ls[NT];
+constexpr int NTR = 1024;
+__global__ __launch_bounds__(NTR) void k_reduce(const double *__restrict__ partials, int nb,
+                                                double *__restrict__ out) {
+  __shared__ double hs[NTR], ls[NTR];
   const double *row = partials + (size_t)blockIdx.x * nb;
-  double hi = 0.0, lo = 0.0;
-  for (int i = threadIdx.x; i < nb; i += NT) {
+  // four independent compensated accumulators per thread keep four loads in
+  // flight; they are combined in a fixed order
+  double h4[4] = {0.0, 0.0, 0.0, 0.0}, l4[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < nb; i += 4 * NTR) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (i + k * NTR < nb) ? row[i + k * NTR] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double e;
+      two_sum(h4[k], v[k], h4[k], e);
+      l4[k] += e;
+    }
+  }
+  double hi = h4[0], lo = l4[0];
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
     double e;
-    two_sum(hi, row[i], hi, e);
-    lo += e;
+    two_sum(hi, h4[k], hi, e);
+    lo += l4[k] + e;
   }
   hs[threadIdx.x] = hi;
   ls[threadIdx.x] = lo;
   __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
+  for (int w = NTR / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) {
       double e, h;
       two_sum(hs[threadIdx.x], hs[threadIdx.x + w], h, e);
@@ -394,10 +413,59 @@ int fused_mode(const Geo &g) {
   return zs_ok ? 2 : 1;
 }
 
+// SU2GPU_FAST = 0 (generic kernels only) | 1 (specialised SU(2) d=4 step for
+// FP32, default) | 2 (also for FP64); SU2GPU_TBLOCK = 0 disables the t-blocked
+// XCD layout of the specialised step
+static int env_int(const char *name, int def) {
+  const char *e = getenv(name);
+  return e ? atoi(e) : def;
+}
+static int fast_level() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_FAST", 1);
+  return v;
+}
+static int tblock_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_TBLOCK", 1);
+  return v;
+}
+
+template <class G, int ND>
+static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
+  using T = typename G::real;
+  if constexpr (ND != 4 || !std::is_same<G, GSU2<T>>::value) {
+    return false;
+  } else {
+    const int lvl = fast_level();
+    if (lvl == 0 || (sizeof(T) == 8 && lvl < 2)) return false;
+    if (L.c.flags & (F_PZERO | F_ROUND | F_ANISO)) return false;
+    // the thread-per-site shape keeps its own partial-sum count (fused_blocks)
+    if (fused_mode(L.geo) == 0) return false;
+    FastGeo f;
+    if (!make_fast_geo(L.geo.L, L.geo.V, tblock_pref() != 0, f)) return false;
+    dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
+    const bool acc = (L.c.flags & F_ACCUM) != 0;
+#define SU2_FAST(W, A)                                                                        \
+  hipLaunchKernelGGL((k_md_su2_4d<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin,     \
+                     (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride)
+    if (L.Uout) {
+      if (acc) SU2_FAST(true, true);
+      else SU2_FAST(true, false);
+    } else {
+      if (acc) SU2_FAST(false, true);
+      else SU2_FAST(false, false);
+    }
+#undef SU2_FAST
+    return true;
+  }
+}
+
 template <class G, int ND>
 static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
   using link = typename G::link;
   using T = typename G::real;
+  if (launch_fast<G, ND>(L, st)) return;
   const int mode = fused_mode(L.geo);
   if (mode != 0) {
     dim3 grid(ND * nblk(L.geo.V)), block(NT);
@@ -520,7 +588,7 @@ void launch_scale(int prec, void *P, long n, double f, hipStream_t st) {
 }
 
 void launch_reduce(const double *partials, int nb, int nrows, double *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce, dim3(nrows), dim3(NT), 0, st, partials, nb, out);
+  hipLaunchKernelGGL(k_reduce, dim3(nrows), dim3(NTR), 0, st, partials, nb, out);
 }
 
 template <class GS>

@@ -1,0 +1,327 @@
+// su2_md_fast.hpp -- the fused force / momentum / link update of the HMC hot
+// loop, specialised for SU(2) in d = 4 (get_staples.hh:26-80,
+// flat-gaugemonomial.hh:105-131, update_momenta.hh:27-30, update_gauge.hh:11-14).
+//
+// The generic kernels (su2_md_site.hpp) are limited by VALU issue, not by HBM:
+// a thread per link spends ~800 vector instructions on integer division for
+// its coordinates, 64-bit address arithmetic, TwoSum-compensated staple sums
+// and the library sin/cos.  This body keeps the arithmetic of the step and
+// removes the rest:
+//   * coordinates come from the launch layout (scalar block decomposition,
+//     one multiply-high division per coordinate), neighbour offsets along t
+//     and x are wave-uniform;
+//   * every load is a per-direction base pointer (scalar) plus a 32-bit
+//     element offset;
+//   * FP32 staples (two quaternion products each, in FP32) are accumulated in
+//     FP64: the sum of the six staples is exact to FP64 rounding, which is a
+//     stronger compensation than TwoSum in FP32 at a third of its cost;
+//   * the force projection and the momentum update run in FP64 on the FP32
+//     operands and round the new momentum once;
+//   * FP32: exp(eps P) = (cos a, sin(a)/a eps P) from the Taylor series of cos
+//     and sin(x)/x in x = a^2 (Horner, truncated at x^7: remainder below
+//     7e-11 for a^2 <= 2.5; larger angles by halving and double-angle formulas);
+//   * FP32: U' = E U in FP64 and reunitarised in FP64 before the single
+//     rounding to FP32 (unbiased; see qrestore<float>).
+// FP64 links keep the generic exp (sincos) and the reference's order of
+// operations for the staple sum, so the FP64 form stays within 1e-10 of the
+// reference per link.
+#pragma once
+#include "su2_md_site.hpp"
+
+namespace su2k {
+
+constexpr int NTF = 256;  // threads per workgroup of the specialised step
+
+// Launch layout of the specialised step.  nb = 4 V / NT blocks; the hardware
+// dispatches block b to XCD b % 8.  With tblock, XCD k owns the t-slices
+// [k Lt/8, (k+1) Lt/8) and walks (3-volume block, t, mu) with mu fastest, so
+// the t +- 1 neighbours of a block were loaded by the blocks just before it on
+// the same XCD and are still in its L2.  Without tblock each XCD owns one
+// contiguous eighth of the (site block, mu) sequence.
+struct FastGeo {
+  int Lt, Lx, Ly, Lz;
+  int plane;      // Ly Lz
+  int vol3;       // Lx Ly Lz
+  int V;
+  int nb3;        // vol3 / NTF
+  int Tc;         // Lt / 8 (tblock)
+  int tblock;
+  uint32_t m_plane, m_Lz, m_Tc, m_nb3;  // multiply-high divisors (fast_div)
+};
+
+// floor(n / d) = umulhi(n, m) with m = floor(2^32 / d) + 1, exact whenever
+// n * d < 2^32 (m d = 2^32 + e with 0 < e <= d, so the error term n e / 2^32
+// stays below 1 / d).  d = 1 has no 32-bit multiplier (2^32 + 1); it is coded
+// as m = 0 and divides by passing n through.
+__host__ __device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return m ? __umulhi(n, m) : n;
+#else
+  return m ? (uint32_t)(((uint64_t)n * m) >> 32) : n;
+#endif
+}
+inline uint32_t fast_div_magic(uint32_t d) {
+  return d == 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / d + 1);
+}
+
+// the link (s, mu) and its periodic neighbour offsets handled by thread tid of
+// hardware block b in a grid of nb = 4 V / NTF blocks
+struct FastSite {
+  int mu, s;
+  int up[4], dn[4];
+};
+__host__ __device__ __forceinline__ FastSite fast_site(const FastGeo &g, uint32_t b, uint32_t nb,
+                                                       uint32_t tid) {
+  FastSite f;
+  int t, blk3;
+  if (g.tblock) {
+    const uint32_t xcd = b & 7u, local = b >> 3;
+    f.mu = (int)(local & 3u);
+    const uint32_t r = local >> 2;
+    const uint32_t q = fast_div(r, g.m_Tc);  // r = q Tc + tt
+    blk3 = (int)q;
+    t = (int)xcd * g.Tc + (int)(r - q * (uint32_t)g.Tc);
+  } else {
+    // each XCD one contiguous eighth of the logical blocks (xcd_block)
+    const uint32_t xcd = b & 7u, local = b >> 3, per = nb >> 3, rem = nb & 7u;
+    const uint32_t lb = xcd * per + (xcd < rem ? xcd : rem) + local;
+    f.mu = (int)(lb & 3u);
+    const uint32_t sb = lb >> 2;
+    const uint32_t q = fast_div(sb, g.m_nb3);
+    t = (int)q;
+    blk3 = (int)(sb - q * (uint32_t)g.nb3);
+  }
+  const uint32_t o3 = (uint32_t)blk3 * NTF + tid;
+  const uint32_t x = fast_div(o3, g.m_plane);
+  const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
+  const uint32_t y = fast_div(rem3, g.m_Lz);
+  const uint32_t z = rem3 - y * (uint32_t)g.Lz;
+  f.s = t * g.vol3 + (int)o3;
+  f.up[0] = (t == g.Lt - 1) ? -(g.Lt - 1) * g.vol3 : g.vol3;
+  f.dn[0] = (t == 0) ? (g.Lt - 1) * g.vol3 : -g.vol3;
+  f.up[1] = ((int)x == g.Lx - 1) ? -(g.Lx - 1) * g.plane : g.plane;
+  f.dn[1] = (x == 0) ? (g.Lx - 1) * g.plane : -g.plane;
+  f.up[2] = ((int)y == g.Ly - 1) ? -(g.Ly - 1) * g.Lz : g.Lz;
+  f.dn[2] = (y == 0) ? (g.Ly - 1) * g.Lz : -g.Lz;
+  f.up[3] = ((int)z == g.Lz - 1) ? -(g.Lz - 1) : 1;
+  f.dn[3] = (z == 0) ? (g.Lz - 1) : -1;
+  return f;
+}
+
+// layout of the specialised step for a d = 4 geometry (extents t, x, y, z), or
+// false where it does not apply: the 3-volume a multiple of the block size,
+// 32-bit byte offsets within one direction plane, and every multiply-high
+// division within its exact range (n d < 2^32)
+inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f) {
+  f.Lt = L[0];
+  f.Lx = L[1];
+  f.Ly = L[2];
+  f.Lz = L[3];
+  f.plane = f.Ly * f.Lz;
+  f.vol3 = f.Lx * f.plane;
+  f.V = V;
+  if (f.vol3 % NTF != 0 || (int64_t)f.Lt * f.vol3 != V) return false;
+  f.nb3 = f.vol3 / NTF;
+  f.tblock = (allow_tblock && f.Lt % 8 == 0) ? 1 : 0;
+  f.Tc = f.tblock ? f.Lt / 8 : 1;
+  const uint64_t two32 = 1ull << 32;
+  if ((uint64_t)V * sizeof(quat<double>) >= two32) return false;
+  if ((uint64_t)f.vol3 * f.plane >= two32 || (uint64_t)f.plane * f.Lz >= two32) return false;
+  const uint64_t nsb = (uint64_t)V / NTF;  // site blocks
+  if (nsb * f.nb3 >= two32 || nsb * f.Tc >= two32) return false;
+  f.m_plane = fast_div_magic(f.plane);
+  f.m_Lz = fast_div_magic(f.Lz);
+  f.m_Tc = fast_div_magic(f.Tc);
+  f.m_nb3 = fast_div_magic(f.nb3);
+  return true;
+}
+
+// element i of an array as scalar base + 32-bit byte offset (global_load with
+// an SGPR base: no 64-bit address arithmetic per load)
+template <typename E> __device__ __forceinline__ E ldo(const E *__restrict__ base, uint32_t i) {
+  return *(const E *)((const char *)base + i * (uint32_t)sizeof(E));
+}
+template <typename E> __device__ __forceinline__ void sto(E *__restrict__ base, uint32_t i, E v) {
+  *(E *)((char *)base + i * (uint32_t)sizeof(E)) = v;
+}
+
+// u v as four FMA chains (16 FMA / MUL), conjugations of the operands passed in
+// through qdag and folded into the source negations of the FMAs
+template <typename T> __device__ __forceinline__ quat<T> qmul_fma(const quat<T> &u, const quat<T> &v) {
+  quat<T> r;
+  r.ar = fma(u.ar, v.ar, -fma(u.ai, v.ai, fma(u.br, v.br, u.bi * v.bi)));
+  r.ai = fma(u.ar, v.ai, fma(u.ai, v.ar, fma(-u.bi, v.br, u.br * v.bi)));
+  r.br = fma(u.ar, v.br, fma(-u.ai, v.bi, fma(u.br, v.ar, u.bi * v.ai)));
+  r.bi = fma(u.ar, v.bi, fma(u.ai, v.br, fma(u.bi, v.ar, -(u.br * v.ai))));
+  return r;
+}
+
+// one staple accumulated into K (FP64)
+template <typename T>
+__device__ __forceinline__ void acc_d(double (&K)[4], const quat<T> &q) {
+  K[0] += double(q.ar);
+  K[1] += double(q.ai);
+  K[2] += double(q.br);
+  K[3] += double(q.bi);
+}
+
+
+// Taylor series of cos(a) and sin(a)/a in x = a^2, Horner form, for x <= 2.5
+__device__ __forceinline__ void cos_sinc_series(double x, double &c, double &s) {
+  c = 1.0 + x * (-1.0 / 2 + x * (1.0 / 24 + x * (-1.0 / 720 + x * (1.0 / 40320 + x * (-1.0 / 3628800 +
+          x * (1.0 / 479001600 + x * (-1.0 / 87178291200.0)))))));
+  s = 1.0 + x * (-1.0 / 6 + x * (1.0 / 120 + x * (-1.0 / 5040 + x * (1.0 / 362880 + x * (-1.0 / 39916800 +
+          x * (1.0 / 6227020800.0 + x * (-1.0 / 1307674368000.0)))))));
+}
+
+// The step for link (s, mu).  MU >= 0: mu = MU is a compile-time constant, the
+// three planes nu != mu are known at compile time and all 18 staple links are
+// loaded before any arithmetic (one memory round trip per thread).  MU < 0:
+// mu at run time, one round trip per plane.
+template <typename T, int MU, bool WRITE_U, bool ACCUM>
+__device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
+                                            quat<T> *__restrict__ Uout, T *__restrict__ P,
+                                            const FastGeo &g, const FusedCoef &c,
+                                            const FastSite &fs, double *__restrict__ partials,
+                                            int pstride, double *lds) {
+  const int mu = MU >= 0 ? MU : fs.mu;
+  const int s = fs.s;
+  const int *up = fs.up, *dn = fs.dn;
+  const int V = g.V;
+  const quat<T> *__restrict__ Um = Uin + (size_t)mu * V;
+  T *__restrict__ P0 = P + (size_t)(mu * 3 + 0) * V;
+  T *__restrict__ P1 = P + (size_t)(mu * 3 + 1) * V;
+  T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
+  const quat<T> u = ldo(Um, (uint32_t)s);
+  const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
+  double K[4] = {0.0, 0.0, 0.0, 0.0};
+  auto staples = [&](const quat<T> &A, const quat<T> &B, const quat<T> &C, const quat<T> &D,
+                     const quat<T> &E, const quat<T> &F) {
+    if (sizeof(T) == 4) {
+      acc_d(K, qmul_fma(qmul_fma(A, qdag(B)), qdag(C)));
+      acc_d(K, qmul_fma(qmul_fma(qdag(D), qdag(E)), F));
+    } else {
+      acc_d(K, GSU2<T>::upper(A, B, C));
+      acc_d(K, GSU2<T>::lower(D, E, F));
+    }
+  };
+  if constexpr (MU >= 0) {
+    quat<T> A[3], B[3], C[3], D[3], E[3], F[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int nu = k + (k >= MU ? 1 : 0);
+      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
+      A[k] = ldo(Un, (uint32_t)(s + up[MU]));           // U_nu(s+mu)
+      B[k] = ldo(Um, (uint32_t)(s + up[nu]));           // U_mu(s+nu)
+      C[k] = ldo(Un, (uint32_t)s);                      // U_nu(s)
+      D[k] = ldo(Un, (uint32_t)(s + up[MU] + dn[nu]));  // U_nu(s+mu-nu)
+      E[k] = ldo(Um, (uint32_t)(s + dn[nu]));           // U_mu(s-nu)
+      F[k] = ldo(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) staples(A[k], B[k], C[k], D[k], E[k], F[k]);
+  } else {
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+      if (nu == mu) continue;
+      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
+      staples(ldo(Un, (uint32_t)(s + up[mu])), ldo(Um, (uint32_t)(s + up[nu])),
+              ldo(Un, (uint32_t)s), ldo(Un, (uint32_t)(s + up[mu] + dn[nu])),
+              ldo(Um, (uint32_t)(s + dn[nu])), ldo(Un, (uint32_t)(s + dn[nu])));
+    }
+  }
+  // ---- force: M = U K (FP64), deriv = 2 (Im b, Re b, Im a) of M
+  const double ua = u.ar, ub = u.ai, uc = u.br, ud = u.bi;
+  const double Mai = ua * K[1] + ub * K[0] - ud * K[2] + uc * K[3];
+  const double Mbr = ua * K[2] - ub * K[3] + uc * K[0] + ud * K[1];
+  const double Mbi = ua * K[3] + ub * K[2] + ud * K[0] - uc * K[1];
+  const double f = 2.0 * c.coef * c.ep;  // P <- P - ep coef 2 Im(M)
+  const T p0 = T(double(pi0) - f * Mbi);
+  const T p1 = T(double(pi1) - f * Mbr);
+  const T p2 = T(double(pi2) - f * Mai);
+  sto(P0, (uint32_t)s, p0);
+  sto(P1, (uint32_t)s, p1);
+  sto(P2, (uint32_t)s, p2);
+  if (WRITE_U) {
+    quat<T> un;
+    if (sizeof(T) == 4) {
+      // exp(eu P) U in FP64 from the stored FP32 momentum, reunitarised in FP64
+      const double e0 = c.eu * double(p0), e1 = c.eu * double(p1), e2 = c.eu * double(p2);
+      const double xx = e0 * e0 + e1 * e1 + e2 * e2;
+      double cs, sn;
+      // beyond the series range halve the angle k times and double back:
+      // cos 2a = 2 cos^2 a - 1, sin(2a)/(2a) = cos a sin(a)/a (never taken
+      // for MD step sizes; a^2 <= 2.5 covers |eps P| <= 1.58)
+      double xr = xx;
+      int k = 0;
+      while (xr > 2.5) {
+        xr *= 0.25;
+        ++k;
+      }
+      cos_sinc_series(xr, cs, sn);
+      for (; k > 0; --k) {
+        sn *= cs;
+        cs = 2.0 * cs * cs - 1.0;
+      }
+      // E = (cos, sinc e2, sinc e1, sinc e0) in (Re a, Im a, Re b, Im b)
+      const double Ea = cs, Eb = sn * e2, Ec = sn * e1, Ed = sn * e0;
+      double na = Ea * ua - Eb * ub - Ec * uc - Ed * ud;
+      double nb = Ea * ub + Eb * ua - Ed * uc + Ec * ud;
+      double nc = Ea * uc - Eb * ud + Ec * ua + Ed * ub;
+      double nd = Ea * ud + Eb * uc + Ed * ua - Ec * ub;
+      if (c.flags & F_RESTORE) {
+        // 1/sqrt(1 + d) = 1 - d/2 + 3 d^2/8 - ..., remainder below 3e-13 for
+        // |d| < 1e-4 (an FP32 link times the series exp has |d| ~ 1e-7)
+        const double d = (na * na + nb * nb + nc * nc + nd * nd) - 1.0;
+        const double r = (fabs(d) < 1e-4) ? 1.0 + d * (-0.5 + 0.375 * d) : 1.0 / sqrt(1.0 + d);
+        na *= r;
+        nb *= r;
+        nc *= r;
+        nd *= r;
+      }
+      un = quat<T>{T(na), T(nb), T(nc), T(nd)};
+    } else {
+      const T pp[3] = {p0, p1, p2};
+      un = GSU2<T>::expmul(pp, T(c.eu), u);
+      if (c.flags & F_RESTORE) un = GSU2<T>::restore(un);
+    }
+    sto(Uout + (size_t)mu * V, (uint32_t)s, un);
+  }
+  if (ACCUM) {
+    const double Mar = ua * K[0] - ub * K[1] - uc * K[2] - ud * K[3];
+    const double ko = double(pi0) * double(pi0) + double(pi1) * double(pi1) + double(pi2) * double(pi2);
+    const double kn = double(p0) * double(p0) + double(p1) * double(p1) + double(p2) * double(p2);
+    const double r0 = block_sum<NTF>(ko, lds);
+    const double r1 = block_sum<NTF>(2.0 * Mar, lds);
+    const double r2 = block_sum<NTF>(kn, lds);
+    if (threadIdx.x == 0) {
+      partials[0 * pstride + blockIdx.x] = r0;
+      partials[1 * pstride + blockIdx.x] = r1;
+      partials[2 * pstride + blockIdx.x] = r2;
+    }
+  }
+}
+
+
+// SPEC: 1 = the block's mu selects one of four bodies with mu fixed at compile
+// time (mu is uniform per block); 0 = one body with mu at run time
+template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ Uin,
+                                                 quat<T> *__restrict__ Uout,
+                                                 T *__restrict__ P, FastGeo g, FusedCoef c,
+                                                 double *__restrict__ partials, int pstride) {
+  __shared__ double lds[NTF / 64];
+  const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
+  if (SPEC) {
+    switch (fs.mu) {
+      case 0: md_su2_body<T, 0, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 1: md_su2_body<T, 1, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 2: md_su2_body<T, 2, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      default: md_su2_body<T, 3, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+    }
+  } else {
+    md_su2_body<T, -1, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
+  }
+}
+
+}  // namespace su2k

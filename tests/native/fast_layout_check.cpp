@@ -1,0 +1,88 @@
+// Host-side check of the launch layout of the specialised SU(2) d=4 step
+// (su2_md_fast.hpp): for each geometry the (block, thread) -> (s, mu) map must
+// hit every link exactly once, and the neighbour offsets must equal the
+// periodic lexicographic neighbours ((t Lx + x) Ly + y) Lz + z.
+// Prints one line per geometry and "ALL OK" when every check passed.
+#include <cstdio>
+#include <vector>
+
+#include "../../su2_amd/csrc/su2_md_fast.hpp"
+
+using namespace su2k;
+
+static int check(const int L[4], bool tblock) {
+  const int V = L[0] * L[1] * L[2] * L[3];
+  FastGeo g;
+  if (!make_fast_geo(L, V, tblock, g)) {
+    printf("%dx%dx%dx%d tblock=%d: not applicable\n", L[0], L[1], L[2], L[3], (int)tblock);
+    return 0;
+  }
+  const uint32_t nb = 4u * (uint32_t)(V / NTF);
+  if (g.tblock && nb % 32 != 0) {
+    printf("FAIL grid %u not a multiple of 32\n", nb);
+    return 1;
+  }
+  std::vector<unsigned char> seen((size_t)4 * V, 0);
+  const int str[4] = {L[1] * L[2] * L[3], L[2] * L[3], L[3], 1};
+  for (uint32_t b = 0; b < nb; ++b) {
+    for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
+      const FastSite f = fast_site(g, b, nb, tid);
+      if (f.mu < 0 || f.mu > 3 || f.s < 0 || f.s >= V) {
+        printf("FAIL out of range b=%u tid=%u mu=%d s=%d\n", b, tid, f.mu, f.s);
+        return 1;
+      }
+      if (seen[(size_t)f.mu * V + f.s]++) {
+        printf("FAIL link (s=%d, mu=%d) visited twice\n", f.s, f.mu);
+        return 1;
+      }
+      int c[4], rest = f.s;
+      for (int d = 0; d < 4; ++d) {
+        c[d] = rest / str[d];
+        rest -= c[d] * str[d];
+      }
+      for (int d = 0; d < 4; ++d) {
+        const int up = f.s + ((c[d] + 1) % L[d] - c[d]) * str[d];
+        const int dn = f.s + ((c[d] + L[d] - 1) % L[d] - c[d]) * str[d];
+        if (f.s + f.up[d] != up || f.s + f.dn[d] != dn) {
+          printf("FAIL neighbours of s=%d along %d\n", f.s, d);
+          return 1;
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < seen.size(); ++i)
+    if (seen[i] != 1) {
+      printf("FAIL link %zu not visited\n", i);
+      return 1;
+    }
+  printf("%dx%dx%dx%d tblock=%d: ok (%u blocks)\n", L[0], L[1], L[2], L[3], g.tblock, nb);
+  return 0;
+}
+
+int main() {
+  // extents (t, x, y, z)
+  const int geoms[][4] = {{8, 8, 8, 8},   {16, 16, 16, 16}, {32, 32, 32, 32}, {24, 24, 24, 24},
+                          {48, 48, 48, 48}, {16, 8, 8, 8},  {12, 8, 8, 8},    {4, 8, 8, 8},
+                          {8, 4, 8, 16},  {8, 16, 16, 1},   {40, 16, 4, 4},   {8, 12, 12, 12},
+                          {64, 16, 16, 16}};
+  int fails = 0;
+  for (const auto &L : geoms)
+    for (int tb = 0; tb < 2; ++tb) fails += check(L, tb != 0);
+  // the multiply-high division against the exact quotient over its stated range
+  for (uint32_t d = 1; d <= 2048; ++d) {
+    const uint32_t m = fast_div_magic(d);
+    const uint32_t nmax = (uint32_t)((1ull << 32) / d);
+    const uint32_t step = nmax / 4099 + 1;
+    for (uint64_t n = 0; n < nmax; n += step)
+      if (fast_div((uint32_t)n, m) != (uint32_t)n / d) {
+        printf("FAIL fast_div %u / %u\n", (uint32_t)n, d);
+        return 1;
+      }
+    if (fast_div(nmax - 1, m) != (nmax - 1) / d) {
+      printf("FAIL fast_div %u / %u\n", nmax - 1, d);
+      return 1;
+    }
+  }
+  printf(fails ? "FAILED\n" : "ALL OK\n");
+  return fails ? 1 : 0;
+}

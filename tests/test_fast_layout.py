@@ -1,0 +1,26 @@
+"""CPU check of the launch layout of the specialised SU(2) d=4 MD step: the
+(block, thread) -> link map is a bijection onto the 4 V links with periodic
+lexicographic neighbours, and the multiply-high division is exact over its
+stated range.  Compiles tests/native/fast_layout_check.cpp (host code only runs)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not available")
+def test_fast_step_layout(tmp_path):
+    exe = tmp_path / "fast_layout_check"
+    src = os.path.join(HERE, "native", "fast_layout_check.cpp")
+    subprocess.run([HIPCC, "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17",
+                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                    "-o", str(exe), src], check=True, capture_output=True, timeout=600)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip().endswith("ALL OK")
+    assert "32x32x32x32 tblock=1: ok" in out.stdout
+    assert "8x8x8x8 tblock=1: ok" in out.stdout
