@@ -242,7 +242,7 @@ def measured_traffic(L, form):
     counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor that
     the same counter reads for a copy of known size); None if no summary for
     this lattice size and form is committed."""
-    path = os.path.join(ROOT, "profiles", "k_md_link_traffic.json")
+    path = os.path.join(ROOT, "profiles", "fused_step_traffic.json")
     try:
         with open(path) as f:
             tab = json.load(f)
@@ -268,6 +268,7 @@ def main():
     ap.add_argument("--prec-beta", type=float, default=2.5, help="beta of the FP32-vs-FP64 study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-flow", action="store_true", help="skip the flow-adjoint leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 study")
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -279,8 +280,8 @@ def main():
     S.load_library()
     f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
     f64 = run_form(S, S.FP64, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
-    ddh = delta_dH(S, args.prec_L, args.nsteps, args.prec_beta, args.seed, args.ddh_traj,
-                   nchain=args.chain)
+    ddh = None if args.no_precision else delta_dH(S, args.prec_L, args.nsteps, args.prec_beta,
+                                                   args.seed, args.ddh_traj, nchain=args.chain)
     copy_ms = S.copy_probe(1 << 30, 20)
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
@@ -289,7 +290,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
     n = args.nsteps
-    kname = "k_md_link<GSU2<float>,4,true,*>"
+    kname = "k_md_su2_4d<float,true,false,1,false>"
     out = {
         "metric": "SU(2) 32^4 HMC link-updates/s, %HBM roofline; FP32 vs FP64 form at |dDH|<=1e-6",
         "value": f32["link_updates_per_s"],

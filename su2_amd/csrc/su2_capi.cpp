@@ -955,6 +955,7 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
   const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
   const int pflags = h->aniso ? su2k::F_ANISO : 0;
   h->cur = u_start;
+  // the FP32 form takes S_old as the FP64 plaquette sum of its FP32 links
   if (h->prec == 1) CK(plaq_launch(h, u_start, -1, w_t, w_s, 0, pflags, rbase + 6));
   IntegrateOut io;
   CK(run_integrator(h, kind, nsteps, tau, exponent, protect, true, rbase, &io));

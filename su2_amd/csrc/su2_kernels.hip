@@ -430,6 +430,12 @@ static int tblock_pref() {
   if (v < 0) v = env_int("SU2GPU_TBLOCK", 1);
   return v;
 }
+// SU2GPU_FASTSHAPE = 0: one thread per link (default), 1: one thread per site
+static int fast_shape() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_FASTSHAPE", 0);
+  return v;
+}
 
 template <class G, int ND>
 static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
@@ -444,11 +450,16 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     if (fused_mode(L.geo) == 0) return false;
     FastGeo f;
     if (!make_fast_geo(L.geo.L, L.geo.V, tblock_pref() != 0, f)) return false;
-    dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
     const bool acc = (L.c.flags & F_ACCUM) != 0;
-#define SU2_FAST(W, A)                                                                        \
-  hipLaunchKernelGGL((k_md_su2_4d<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin,     \
-                     (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride)
+    const bool site = fast_shape() == 1;
+    dim3 grid((site ? 1 : 4) * (L.geo.V / NTF)), block(NTF);
+#define SU2_FAST(W, A)                                                                          \
+  if (site)                                                                                     \
+    hipLaunchKernelGGL((k_md_su2_4d_site<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin, \
+                       (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride);             \
+  else                                                                                          \
+    hipLaunchKernelGGL((k_md_su2_4d<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin,      \
+                       (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride)
     if (L.Uout) {
       if (acc) SU2_FAST(true, true);
       else SU2_FAST(true, false);
@@ -559,6 +570,26 @@ static void launch_plaq_g(const void *Uin, void *Uout, const Geo &g, double w_t,
 }
 void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
                  double w_s, int mu_start, int flags, double *partials, hipStream_t st) {
+  FastGeo f;
+  if (group == 1 && g.nd == 4 && fast_level() > 0 &&
+      make_fast_geo(g.L, g.V, tblock_pref() != 0, f)) {
+    // specialised SU(2) d=4 sum: V / NTF site blocks (= site_blocks(g))
+    const dim3 grid(g.V / NTF), block(NTF);
+    const bool rs = (flags & F_RESTORE) != 0;
+    const int an = (flags & F_ANISO) ? 1 : 0;
+#define SU2_PLAQ(T, R)                                                                     \
+  hipLaunchKernelGGL((k_plaq_su2_4d<T, R>), grid, block, 0, st, (const quat<T> *)Uin,     \
+                     (quat<T> *)Uout, f, w_t, w_s, mu_start, an, partials)
+    if (prec == 0) {
+      if (rs) SU2_PLAQ(double, true);
+      else SU2_PLAQ(double, false);
+    } else {
+      if (rs) SU2_PLAQ(float, true);
+      else SU2_PLAQ(float, false);
+    }
+#undef SU2_PLAQ
+    return;
+  }
   if (group == 1) {
     if (prec == 0) launch_plaq_g<GSU2<double>, GSU2<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
     else launch_plaq_g<GSU2<float>, GSU2<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);

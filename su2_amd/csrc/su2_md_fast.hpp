@@ -174,74 +174,29 @@ __device__ __forceinline__ void cos_sinc_series(double x, double &c, double &s) 
           x * (1.0 / 6227020800.0 + x * (-1.0 / 1307674368000.0)))))));
 }
 
-// The step for link (s, mu).  MU >= 0: mu = MU is a compile-time constant, the
-// three planes nu != mu are known at compile time and all 18 staple links are
-// loaded before any arithmetic (one memory round trip per thread).  MU < 0:
-// mu at run time, one round trip per plane.
-template <typename T, int MU, bool WRITE_U, bool ACCUM>
-__device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
-                                            quat<T> *__restrict__ Uout, T *__restrict__ P,
-                                            const FastGeo &g, const FusedCoef &c,
-                                            const FastSite &fs, double *__restrict__ partials,
-                                            int pstride, double *lds) {
-  const int mu = MU >= 0 ? MU : fs.mu;
-  const int s = fs.s;
-  const int *up = fs.up, *dn = fs.dn;
-  const int V = g.V;
-  const quat<T> *__restrict__ Um = Uin + (size_t)mu * V;
-  T *__restrict__ P0 = P + (size_t)(mu * 3 + 0) * V;
-  T *__restrict__ P1 = P + (size_t)(mu * 3 + 1) * V;
-  T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
-  const quat<T> u = ldo(Um, (uint32_t)s);
-  const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
-  double K[4] = {0.0, 0.0, 0.0, 0.0};
-  auto staples = [&](const quat<T> &A, const quat<T> &B, const quat<T> &C, const quat<T> &D,
-                     const quat<T> &E, const quat<T> &F) {
-    if (sizeof(T) == 4) {
-      acc_d(K, qmul_fma(qmul_fma(A, qdag(B)), qdag(C)));
-      acc_d(K, qmul_fma(qmul_fma(qdag(D), qdag(E)), F));
-    } else {
-      acc_d(K, GSU2<T>::upper(A, B, C));
-      acc_d(K, GSU2<T>::lower(D, E, F));
-    }
-  };
-  if constexpr (MU >= 0) {
-    quat<T> A[3], B[3], C[3], D[3], E[3], F[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int nu = k + (k >= MU ? 1 : 0);
-      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
-      A[k] = ldo(Un, (uint32_t)(s + up[MU]));           // U_nu(s+mu)
-      B[k] = ldo(Um, (uint32_t)(s + up[nu]));           // U_mu(s+nu)
-      C[k] = ldo(Un, (uint32_t)s);                      // U_nu(s)
-      D[k] = ldo(Un, (uint32_t)(s + up[MU] + dn[nu]));  // U_nu(s+mu-nu)
-      E[k] = ldo(Um, (uint32_t)(s + dn[nu]));           // U_mu(s-nu)
-      F[k] = ldo(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) staples(A[k], B[k], C[k], D[k], E[k], F[k]);
-  } else {
-#pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
-      if (nu == mu) continue;
-      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
-      staples(ldo(Un, (uint32_t)(s + up[mu])), ldo(Um, (uint32_t)(s + up[nu])),
-              ldo(Un, (uint32_t)s), ldo(Un, (uint32_t)(s + up[mu] + dn[nu])),
-              ldo(Um, (uint32_t)(s + dn[nu])), ldo(Un, (uint32_t)(s + dn[nu])));
-    }
-  }
-  // ---- force: M = U K (FP64), deriv = 2 (Im b, Re b, Im a) of M
+// Momentum and link update of one link from its staple sum K (FP64):
+// M = U K, P <- P - ep coef 2 Im(M) rounded once, U' = exp(eu P) U.
+// Returns Re(M) and the new momentum for the block sums.
+template <typename T, bool WRITE_U>
+__device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (&K)[4], T pi0,
+                                                T pi1, T pi2, T *__restrict__ P0,
+                                                T *__restrict__ P1, T *__restrict__ P2,
+                                                quat<T> *__restrict__ Uo, uint32_t s,
+                                                const FusedCoef &c, T (&pn)[3]) {
   const double ua = u.ar, ub = u.ai, uc = u.br, ud = u.bi;
   const double Mai = ua * K[1] + ub * K[0] - ud * K[2] + uc * K[3];
   const double Mbr = ua * K[2] - ub * K[3] + uc * K[0] + ud * K[1];
   const double Mbi = ua * K[3] + ub * K[2] + ud * K[0] - uc * K[1];
-  const double f = 2.0 * c.coef * c.ep;  // P <- P - ep coef 2 Im(M)
+  const double f = 2.0 * c.coef * c.ep;
   const T p0 = T(double(pi0) - f * Mbi);
   const T p1 = T(double(pi1) - f * Mbr);
   const T p2 = T(double(pi2) - f * Mai);
-  sto(P0, (uint32_t)s, p0);
-  sto(P1, (uint32_t)s, p1);
-  sto(P2, (uint32_t)s, p2);
+  sto(P0, s, p0);
+  sto(P1, s, p1);
+  sto(P2, s, p2);
+  pn[0] = p0;
+  pn[1] = p1;
+  pn[2] = p2;
   if (WRITE_U) {
     quat<T> un;
     if (sizeof(T) == 4) {
@@ -281,19 +236,121 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
       }
       un = quat<T>{T(na), T(nb), T(nc), T(nd)};
     } else {
-      const T pp[3] = {p0, p1, p2};
-      un = GSU2<T>::expmul(pp, T(c.eu), u);
+      un = GSU2<T>::expmul(pn, T(c.eu), u);
       if (c.flags & F_RESTORE) un = GSU2<T>::restore(un);
     }
-    sto(Uout + (size_t)mu * V, (uint32_t)s, un);
+    sto(Uo, s, un);
   }
+  return ua * K[0] - ub * K[1] - uc * K[2] - ud * K[3];
+}
+
+template <typename T> __device__ __forceinline__ double sq3(T a, T b, T c) {
+  return double(a) * double(a) + double(b) * double(b) + double(c) * double(c);
+}
+
+// upper + lower staple of one plane into K (FP64): the reference's
+// U_nu(s+mu) U_mu(s+nu)^+ U_nu(s)^+ and U_nu(s+mu-nu)^+ U_mu(s-nu)^+ U_nu(s-nu)
+template <typename T>
+__device__ __forceinline__ void staple_pair(double (&K)[4], const quat<T> &A, const quat<T> &B,
+                                            const quat<T> &C, const quat<T> &D, const quat<T> &E,
+                                            const quat<T> &F) {
+  if (sizeof(T) == 4) {
+    acc_d(K, qmul_fma(qmul_fma(A, qdag(B)), qdag(C)));
+    acc_d(K, qmul_fma(qmul_fma(qdag(D), qdag(E)), F));
+  } else {
+    acc_d(K, GSU2<T>::upper(A, B, C));
+    acc_d(K, GSU2<T>::lower(D, E, F));
+  }
+}
+
+// The step for link (s, mu).  MU >= 0: mu = MU is a compile-time constant, the
+// three planes nu != mu are known at compile time and all 18 staple links are
+// loaded before any arithmetic (one memory round trip per thread).  MU < 0:
+// mu at run time, one round trip per plane.
+template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false>
+__device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
+                                            quat<T> *__restrict__ Uout, T *__restrict__ P,
+                                            const FastGeo &g, const FusedCoef &c,
+                                            const FastSite &fs, double *__restrict__ partials,
+                                            int pstride, double *lds) {
+  const int mu = MU >= 0 ? MU : fs.mu;
+  const int s = fs.s;
+  const int *up = fs.up, *dn = fs.dn;
+  const int V = g.V;
+  const quat<T> *__restrict__ Um = Uin + (size_t)mu * V;
+  T *__restrict__ P0 = P + (size_t)(mu * 3 + 0) * V;
+  T *__restrict__ P1 = P + (size_t)(mu * 3 + 1) * V;
+  T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
+  const quat<T> u = ldo(Um, (uint32_t)s);
+  const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
+  double K[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (MU >= 0) {
+    quat<T> A[3], B[3], C[3], D[3], E[3], F[3];
+    // ZS: the +-z neighbours of the wave's sites sit in the neighbouring lanes
+    // (the wave holds whole z-rows), so they are exchanged through ds_bpermute
+    // instead of loaded: 4 of the 6 loads of the plane nu = z, 2 of 6 in every
+    // plane when mu = z
+    int zup = 0, zdn = 0;
+    if (ZS) {
+      const int lane = threadIdx.x & 63, zl = lane & (g.Lz - 1);
+      zup = (zl == g.Lz - 1) ? lane - (g.Lz - 1) : lane + 1;
+      zdn = (zl == 0) ? lane + (g.Lz - 1) : lane - 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int nu = k + (k >= MU ? 1 : 0);
+      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
+      if (ZS && nu == 3) {
+        A[k] = ldo(Un, (uint32_t)(s + up[MU]));         // U_z(s+mu)
+        C[k] = ldo(Un, (uint32_t)s);                    // U_z(s)
+      } else if (ZS && MU == 3) {
+        B[k] = ldo(Um, (uint32_t)(s + up[nu]));         // U_z(s+nu)
+        C[k] = ldo(Un, (uint32_t)s);                    // U_nu(s)
+        E[k] = ldo(Um, (uint32_t)(s + dn[nu]));         // U_z(s-nu)
+        F[k] = ldo(Un, (uint32_t)(s + dn[nu]));         // U_nu(s-nu)
+      } else {
+        A[k] = ldo(Un, (uint32_t)(s + up[MU]));           // U_nu(s+mu)
+        B[k] = ldo(Um, (uint32_t)(s + up[nu]));           // U_mu(s+nu)
+        C[k] = ldo(Un, (uint32_t)s);                      // U_nu(s)
+        D[k] = ldo(Un, (uint32_t)(s + up[MU] + dn[nu]));  // U_nu(s+mu-nu)
+        E[k] = ldo(Um, (uint32_t)(s + dn[nu]));           // U_mu(s-nu)
+        F[k] = ldo(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
+      }
+    }
+    if (ZS) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int nu = k + (k >= MU ? 1 : 0);
+        if (nu == 3) {
+          B[k] = qshfl(u, zup);     // U_mu(s+z)
+          E[k] = qshfl(u, zdn);     // U_mu(s-z)
+          D[k] = qshfl(A[k], zdn);  // U_z(s+mu-z)
+          F[k] = qshfl(C[k], zdn);  // U_z(s-z)
+        } else if (MU == 3) {
+          A[k] = qshfl(C[k], zup);  // U_nu(s+z)
+          D[k] = qshfl(F[k], zup);  // U_nu(s+z-nu)
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) staple_pair(K, A[k], B[k], C[k], D[k], E[k], F[k]);
+  } else {
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+      if (nu == mu) continue;
+      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
+      staple_pair(K, ldo(Un, (uint32_t)(s + up[mu])), ldo(Um, (uint32_t)(s + up[nu])),
+                  ldo(Un, (uint32_t)s), ldo(Un, (uint32_t)(s + up[mu] + dn[nu])),
+                  ldo(Um, (uint32_t)(s + dn[nu])), ldo(Un, (uint32_t)(s + dn[nu])));
+    }
+  }
+  T pn[3];
+  const double mar = md_su2_update<T, WRITE_U>(u, K, pi0, pi1, pi2, P0, P1, P2,
+                                               Uout + (size_t)mu * V, (uint32_t)s, c, pn);
   if (ACCUM) {
-    const double Mar = ua * K[0] - ub * K[1] - uc * K[2] - ud * K[3];
-    const double ko = double(pi0) * double(pi0) + double(pi1) * double(pi1) + double(pi2) * double(pi2);
-    const double kn = double(p0) * double(p0) + double(p1) * double(p1) + double(p2) * double(p2);
-    const double r0 = block_sum<NTF>(ko, lds);
-    const double r1 = block_sum<NTF>(2.0 * Mar, lds);
-    const double r2 = block_sum<NTF>(kn, lds);
+    const double r0 = block_sum<NTF>(sq3(pi0, pi1, pi2), lds);
+    const double r1 = block_sum<NTF>(2.0 * mar, lds);
+    const double r2 = block_sum<NTF>(sq3(pn[0], pn[1], pn[2]), lds);
     if (threadIdx.x == 0) {
       partials[0 * pstride + blockIdx.x] = r0;
       partials[1 * pstride + blockIdx.x] = r1;
@@ -302,10 +359,10 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
   }
 }
 
-
 // SPEC: 1 = the block's mu selects one of four bodies with mu fixed at compile
 // time (mu is uniform per block); 0 = one body with mu at run time
-template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1>
+// ZS (with SPEC; needs 64 % Lz == 0): +-z neighbours by lane exchange
+template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ Uin,
                                                  quat<T> *__restrict__ Uout,
                                                  T *__restrict__ P, FastGeo g, FusedCoef c,
@@ -314,14 +371,179 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ U
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
   if (SPEC) {
     switch (fs.mu) {
-      case 0: md_su2_body<T, 0, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 1: md_su2_body<T, 1, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 2: md_su2_body<T, 2, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      default: md_su2_body<T, 3, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
     }
   } else {
     md_su2_body<T, -1, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
   }
+}
+
+// The same step with one thread per SITE updating its four links.  The upper
+// staples of links mu and nu in plane (mu, nu) share W = U_nu(s+mu) U_mu(s+nu)^+:
+//   upper_mu = W U_nu(s)^+,  upper_nu = (U_mu(s) W)^+,
+// so a site needs 42 products (10.5 per link instead of 12) and 52 loads
+// (13 per link instead of 19).  Launch: V / NTF blocks, sites from fast_site_s
+// (the link layout without mu); the block sums of block b go to partial entry
+// 4 b and entries 4 b + 1..3 are written 0, so the partial count (4 V / NTF)
+// equals the link shape's.
+__host__ __device__ __forceinline__ void fast_site_s(const FastGeo &g, uint32_t b, uint32_t nb,
+                                                     uint32_t tid, int &s, int (&up)[4],
+                                                     int (&dn)[4]) {
+  int t, blk3;
+  if (g.tblock) {
+    const uint32_t xcd = b & 7u, local = b >> 3;
+    const uint32_t q = fast_div(local, g.m_Tc);
+    blk3 = (int)q;
+    t = (int)xcd * g.Tc + (int)(local - q * (uint32_t)g.Tc);
+  } else {
+    const uint32_t xcd = b & 7u, local = b >> 3, per = nb >> 3, rem = nb & 7u;
+    const uint32_t lb = xcd * per + (xcd < rem ? xcd : rem) + local;
+    const uint32_t q = fast_div(lb, g.m_nb3);
+    t = (int)q;
+    blk3 = (int)(lb - q * (uint32_t)g.nb3);
+  }
+  const uint32_t o3 = (uint32_t)blk3 * NTF + tid;
+  const uint32_t x = fast_div(o3, g.m_plane);
+  const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
+  const uint32_t y = fast_div(rem3, g.m_Lz);
+  const uint32_t z = rem3 - y * (uint32_t)g.Lz;
+  s = t * g.vol3 + (int)o3;
+  up[0] = (t == g.Lt - 1) ? -(g.Lt - 1) * g.vol3 : g.vol3;
+  dn[0] = (t == 0) ? (g.Lt - 1) * g.vol3 : -g.vol3;
+  up[1] = ((int)x == g.Lx - 1) ? -(g.Lx - 1) * g.plane : g.plane;
+  dn[1] = (x == 0) ? (g.Lx - 1) * g.plane : -g.plane;
+  up[2] = ((int)y == g.Ly - 1) ? -(g.Ly - 1) * g.Lz : g.Lz;
+  dn[2] = (y == 0) ? (g.Ly - 1) * g.Lz : -g.Lz;
+  up[3] = ((int)z == g.Lz - 1) ? -(g.Lz - 1) : 1;
+  dn[3] = (z == 0) ? (g.Lz - 1) : -1;
+}
+
+template <typename T, bool WRITE_U, bool ACCUM>
+__device__ __forceinline__ void md_su2_site_body(const quat<T> *__restrict__ Uin,
+                                                 quat<T> *__restrict__ Uout, T *__restrict__ P,
+                                                 const FastGeo &g, const FusedCoef &c,
+                                                 double *__restrict__ partials, int pstride,
+                                                 double *lds) {
+  int s, up[4], dn[4];
+  fast_site_s(g, blockIdx.x, gridDim.x, threadIdx.x, s, up, dn);
+  const int V = g.V;
+  quat<T> own[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) own[m] = ldo(Uin + (size_t)m * V, (uint32_t)s);
+  double K[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) K[m][0] = K[m][1] = K[m][2] = K[m][3] = 0.0;
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+#pragma unroll
+    for (int nu = mu + 1; nu < 4; ++nu) {
+      const quat<T> *__restrict__ Um = Uin + (size_t)mu * V;
+      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
+      const quat<T> A = ldo(Un, (uint32_t)(s + up[mu]));            // U_nu(s+mu)
+      const quat<T> B = ldo(Um, (uint32_t)(s + up[nu]));            // U_mu(s+nu)
+      const quat<T> Dm = ldo(Un, (uint32_t)(s + up[mu] + dn[nu]));  // U_nu(s+mu-nu)
+      const quat<T> Em = ldo(Um, (uint32_t)(s + dn[nu]));           // U_mu(s-nu)
+      const quat<T> Fm = ldo(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
+      const quat<T> Dn = ldo(Um, (uint32_t)(s + up[nu] + dn[mu]));  // U_mu(s+nu-mu)
+      const quat<T> En = ldo(Un, (uint32_t)(s + dn[mu]));           // U_nu(s-mu)
+      const quat<T> Fn = ldo(Um, (uint32_t)(s + dn[mu]));           // U_mu(s-mu)
+      const quat<T> W = qmul_fma(A, qdag(B));
+      acc_d(K[mu], qmul_fma(W, qdag(own[nu])));
+      acc_d(K[nu], qdag(qmul_fma(own[mu], W)));
+      acc_d(K[mu], qmul_fma(qmul_fma(qdag(Dm), qdag(Em)), Fm));
+      acc_d(K[nu], qmul_fma(qmul_fma(qdag(Dn), qdag(En)), Fn));
+      // keep the scheduler from hoisting the next plane's loads above this
+      // plane's arithmetic (register pressure)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  double ko = 0.0, act = 0.0, kn = 0.0;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    T *__restrict__ P0 = P + (size_t)(mu * 3 + 0) * V;
+    T *__restrict__ P1 = P + (size_t)(mu * 3 + 1) * V;
+    T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
+    const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
+    T pn[3];
+    const double mar = md_su2_update<T, WRITE_U>(own[mu], K[mu], pi0, pi1, pi2, P0, P1, P2,
+                                                 Uout + (size_t)mu * V, (uint32_t)s, c, pn);
+    if (ACCUM) {
+      ko += sq3(pi0, pi1, pi2);
+      act += 2.0 * mar;
+      kn += sq3(pn[0], pn[1], pn[2]);
+    }
+  }
+  if (ACCUM) {
+    const double r0 = block_sum<NTF>(ko, lds);
+    const double r1 = block_sum<NTF>(act, lds);
+    const double r2 = block_sum<NTF>(kn, lds);
+    if (threadIdx.x < 4) {
+      const uint32_t e = 4u * blockIdx.x + threadIdx.x;
+      partials[0 * pstride + e] = threadIdx.x == 0 ? r0 : 0.0;
+      partials[1 * pstride + e] = threadIdx.x == 0 ? r1 : 0.0;
+      partials[2 * pstride + e] = threadIdx.x == 0 ? r2 : 0.0;
+    }
+  }
+}
+
+template <typename T, bool WRITE_U, bool ACCUM>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_site(const quat<T> *__restrict__ Uin,
+                                                      quat<T> *__restrict__ Uout,
+                                                      T *__restrict__ P, FastGeo g, FusedCoef c,
+                                                      double *__restrict__ partials, int pstride) {
+  __shared__ double lds[NTF / 64];
+  md_su2_site_body<T, WRITE_U, ACCUM>(Uin, Uout, P, g, c, partials, pstride, lds);
+}
+
+// Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) for SU(2)
+// in d = 4, one thread per site on the site layout of fast_site_s, FP64
+// whatever the storage precision:
+//   ReTr(U_mu(s) U_nu(s+mu) U_mu(s+nu)^+ U_nu(s)^+) = 2 (U_mu(s) U_nu(s+mu)) . (U_nu(s) U_mu(s+nu))
+// (4-vector dot product of two quaternion products).  RESTORE: every link is
+// reunitarised on load and the site's own restored links are written to Uout.
+// Planes with mu < mu_start are skipped; aniso weights w_t (mu = 0) / w_s.
+template <typename T, bool RESTORE>
+__global__ __launch_bounds__(NTF) void k_plaq_su2_4d(const quat<T> *__restrict__ Uin,
+                                                   quat<T> *__restrict__ Uout, FastGeo g,
+                                                   double w_t, double w_s, int mu_start,
+                                                   int aniso, double *__restrict__ partials) {
+  __shared__ double lds[NTF / 64];
+  int s, up[4], dn[4];
+  fast_site_s(g, blockIdx.x, gridDim.x, threadIdx.x, s, up, dn);
+  const int V = g.V;
+  quat<double> own[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    quat<T> q = ldo(Uin + (size_t)m * V, (uint32_t)s);
+    if (RESTORE) {
+      q = qrestore(q);
+      sto(Uout + (size_t)m * V, (uint32_t)s, q);
+    }
+    own[m] = to_d(q);
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int mu = 0; mu < 3; ++mu) {
+    if (mu < mu_start) continue;
+#pragma unroll
+    for (int nu = mu + 1; nu < 4; ++nu) {
+      quat<T> a = ldo(Uin + (size_t)nu * V, (uint32_t)(s + up[mu]));  // U_nu(s+mu)
+      quat<T> b = ldo(Uin + (size_t)mu * V, (uint32_t)(s + up[nu]));  // U_mu(s+nu)
+      if (RESTORE) {
+        a = qrestore(a);
+        b = qrestore(b);
+      }
+      const quat<double> X = qmul(own[mu], to_d(a));
+      const quat<double> Y = qmul(own[nu], to_d(b));
+      const double v = 2.0 * (X.ar * Y.ar + X.ai * Y.ai + X.br * Y.br + X.bi * Y.bi);
+      sum += aniso ? ((mu == 0) ? w_t : w_s) * v : v;
+    }
+  }
+  const double r = block_sum<NTF>(sum, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
 }  // namespace su2k

@@ -55,6 +55,36 @@ static int check(const int L[4], bool tblock) {
       printf("FAIL link %zu not visited\n", i);
       return 1;
     }
+  // the thread-per-site shape: V / NTF blocks, every site exactly once
+  const uint32_t nbs = (uint32_t)(V / NTF);
+  if (g.tblock && nbs % 8 != 0) {
+    printf("FAIL site grid %u not a multiple of 8\n", nbs);
+    return 1;
+  }
+  std::vector<unsigned char> seen_s((size_t)V, 0);
+  for (uint32_t b = 0; b < nbs; ++b) {
+    for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
+      int s, up[4], dn[4];
+      fast_site_s(g, b, nbs, tid, s, up, dn);
+      if (s < 0 || s >= V || seen_s[s]++) {
+        printf("FAIL site map b=%u tid=%u s=%d\n", b, tid, s);
+        return 1;
+      }
+      int c[4], rest = s;
+      for (int d = 0; d < 4; ++d) {
+        c[d] = rest / str[d];
+        rest -= c[d] * str[d];
+      }
+      for (int d = 0; d < 4; ++d) {
+        const int upn = s + ((c[d] + 1) % L[d] - c[d]) * str[d];
+        const int dnn = s + ((c[d] + L[d] - 1) % L[d] - c[d]) * str[d];
+        if (s + up[d] != upn || s + dn[d] != dnn) {
+          printf("FAIL site neighbours of s=%d along %d\n", s, d);
+          return 1;
+        }
+      }
+    }
+  }
   printf("%dx%dx%dx%d tblock=%d: ok (%u blocks)\n", L[0], L[1], L[2], L[3], g.tblock, nb);
   return 0;
 }

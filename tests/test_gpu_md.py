@@ -265,6 +265,28 @@ def test_fp32_form_tracks_fp64(S, oracle):
     assert np.abs(Ug - Uw).max() < 1e-4
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8, 4), (16, 8, 8, 8, 4), (4, 4, 4, 4, 4)])
+@pytest.mark.parametrize("kind", [6, 0])
+def test_fp32_trajectory_action_sums(S, oracle, dims, kind):
+    """S_old / S_new of an FP32 trajectory (summed inside the first and last
+    fused launches where the specialised step applies, by the plaquette kernel
+    elsewhere) equal the plaquette-kernel action of the start and final links."""
+    U = _start(oracle, 1, dims)
+    P = oracle.initnormal(1, dims, SEED + 3)
+    with S.Lattice(S.SU2, dims, 2.4, precision=S.FP32) as lat:
+        lat.set_links(U)
+        s_start = lat.gauge_action()
+        r = lat.md_trajectory(P, kind, 10)
+        lat.md_finish(True)
+        s_end = lat.gauge_action()
+        Uend = lat.get_links()
+    assert r["gauge_old"] == pytest.approx(s_start, rel=1e-12)
+    assert r["gauge_new"] == pytest.approx(s_end, rel=1e-12)
+    # the final links are unit quaternions (reunitarised in every link update)
+    q = Uend.reshape(-1, 4)
+    assert np.abs((q.astype(np.float64) ** 2).sum(1) - 1.0).max() < 5e-7
+
+
 def test_device_momenta_statistics(S):
     dims = (8, 8, 8, 8, 4)
     with S.Lattice(S.SU2, dims, 2.4) as lat:
