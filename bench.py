@@ -228,9 +228,20 @@ def cpu_baseline(L, nsteps, beta, seed):
     kind = "reference" if orc.have_reference() else "port"
     lib = orc.Reference() if kind == "reference" else orc.Oracle()
     U = orc.Oracle().haarstart(orc.SU2, dims, seed)
-    t0 = time.perf_counter()
-    lib.md_update(orc.SU2, dims, U, beta, orc.OMF2, nsteps, seed)
-    dt = time.perf_counter() - t0
+    # the reference's set_integrator prints the integrator's name on stdout;
+    # keep this program's stdout to the one JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        t0 = time.perf_counter()
+        lib.md_update(orc.SU2, dims, U, beta, orc.OMF2, nsteps, seed)
+        dt = time.perf_counter() - t0
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
     return {"value": 4 * (L ** 4) * nsteps / dt, "unit": "link-updates/s", "cores": ncores,
             "kind": kind,
             "sample": f"1 OMF2({nsteps}) trajectory of md_update at {L}^4 (incl. momenta, H, copy), "

@@ -277,6 +277,7 @@ int su2gpu_batch_create(su2gpu_batch **out, const su2gpu_batch_member *members, 
     L.geo.str[0] = Ls[3] * Ls[2] * Ls[1];
     L.geo.V = Ls[0] * Ls[1] * Ls[2] * Ls[3];
     L.geo.nd = (int)m.ndims;
+    su2dev::geo_set_div(L.geo);
     L.group = m.group;
     L.kind = m.integrator;
     L.npairs = npairs_of(m.integrator, m.nsteps);

@@ -265,13 +265,14 @@ long double sum_of(const L *h, int r) {
 }
 
 int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, int flags, int r) {
+  int nb;
   {
     Prof pr(h, "plaq");
-    su2k::launch_plaq(h->group, h->prec, h->U[uin], uout >= 0 ? h->U[uout] : nullptr, h->geo,
-                      w_t, w_s, mu_start, flags, row(h, r), h->stream);
+    nb = su2k::launch_plaq(h->group, h->prec, h->U[uin], uout >= 0 ? h->U[uout] : nullptr,
+                           h->geo, w_t, w_s, mu_start, flags, row(h, r), h->stream);
   }
   CK(last_launch("plaq"));
-  return reduce_row(h, r, su2k::site_blocks(h->geo));
+  return reduce_row(h, r, nb);
 }
 
 int sumsq_launch(L *h, const void *P, int r) {
@@ -624,6 +625,7 @@ int su2gpu_create(su2gpu_lattice **out, int group, size_t Lx, size_t Ly, size_t 
   h->geo.str[0] = Ls[3] * Ls[2] * Ls[1];
   h->geo.V = h->V;
   h->geo.nd = h->nd;
+  su2dev::geo_set_div(h->geo);
   hipError_t e = hipGetDevice(&h->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   const int nbf = su2k::fused_blocks(h->geo), nbs = su2k::sumsq_blocks((long)h->nlinks * h->nm);
@@ -961,9 +963,11 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
   CK(run_integrator(h, kind, nsteps, tau, exponent, protect, true, rbase, &io));
   int cur = io.final_buf;
   // restoreSU at the end of integrate (integrator.hh:48,111) fused with the
-  // accept step's action (md_update.hh:47-51): restored links go to a free buffer
+  // accept step's action (md_update.hh:47-51): restored links go to a free
+  // buffer.  The FP32 form reunitarises every link update already (F_RESTORE),
+  // so its final links need no further restore.
   int out;
-  if (h->group == 1) {
+  if (h->group == 1 && h->prec == 0) {
     CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
                     protect.size() > 1 ? *(protect.begin() + 1) : cur},
                 &out));

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace su2dev {
 
 template <typename T> struct alignas(4 * sizeof(T)) quat {
@@ -91,44 +93,85 @@ template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T 
   e = (a - (s - bb)) + (b - bb);
 }
 
+// cos(a) and sin(a)/a in x = a^2: Taylor series in Horner form (remainder
+// below 7e-11 for x <= 2.5), larger angles by halving and the double-angle
+// formulas cos 2a = 2 cos^2 a - 1, sin(2a)/(2a) = cos a sin(a)/a.  Used where
+// the result is rounded to FP32.
+__device__ __forceinline__ void cos_sinc(double x, double &c, double &s) {
+  int k = 0;
+  while (x > 2.5) {
+    x *= 0.25;
+    ++k;
+  }
+  c = 1.0 + x * (-1.0 / 2 + x * (1.0 / 24 + x * (-1.0 / 720 + x * (1.0 / 40320 + x * (-1.0 / 3628800 +
+          x * (1.0 / 479001600 + x * (-1.0 / 87178291200.0)))))));
+  s = 1.0 + x * (-1.0 / 6 + x * (1.0 / 120 + x * (-1.0 / 5040 + x * (1.0 / 362880 + x * (-1.0 / 39916800 +
+          x * (1.0 / 6227020800.0 + x * (-1.0 / 1307674368000.0)))))));
+  for (; k > 0; --k) {
+    s *= c;
+    c = 2.0 * c * c - 1.0;
+  }
+}
+
 // -------------------------------------------------------- group traits ----
 // SU(2): momentum = 3 reals (adjointsu2, adjointfield.hh:12-51); the force
 // projection is get_deriv (adjointfield.hh:53-56) = 2 (Im b, Re b, Im a) of U K.
+// FP32 links: the staples (two FP32 products each) are summed in FP64 and the
+// force is projected in FP64 (exact staple sum; no TwoSum needed), and
+// exp(eps P) U is formed in FP64 from the series above and rounded once.
+// FP64 links keep the reference's order of operations throughout.
 template <typename T> struct GSU2 {
   using real = T;
   using link = quat<T>;
-  using acc = quat<T>;
+  using accr = typename std::conditional<sizeof(T) == 4, double, T>::type;
+  using acc = quat<accr>;
   static constexpr int NMOM = 3;
   static constexpr int NCOMP = 4;
   static constexpr double Nc = 2.0;
   __device__ static acc zero() { return acc{0, 0, 0, 0}; }
+  __device__ static acc widen(const quat<T> &q) {
+    return acc{accr(q.ar), accr(q.ai), accr(q.br), accr(q.bi)};
+  }
   // U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+   (get_staples.hh:38-45)
   __device__ static acc upper(const link &a, const link &b, const link &c) {
-    return qmul(qmul(a, qdag(b)), qdag(c));
+    return widen(qmul(qmul(a, qdag(b)), qdag(c)));
   }
   // U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)   (get_staples.hh:46-54)
   __device__ static acc lower(const link &a, const link &b, const link &c) {
-    return qmul(qmul(qdag(a), qdag(b)), c);
+    return widen(qmul(qmul(qdag(a), qdag(b)), c));
   }
   __device__ static acc add(const acc &x, const acc &y) { return qadd(x, y); }
-  __device__ static acc scale(T f, const acc &x) { return qscale(f, x); }
-  // compensated accumulation: s + c holds the running sum
+  __device__ static acc scale(T f, const acc &x) { return qscale(accr(f), x); }
+  // compensated accumulation: s + c holds the running sum (FP32 links: the
+  // FP64 sum is already exact to FP64 rounding)
   __device__ static void add_comp(acc &s, acc &c, const acc &x) {
-    T e;
-    two_sum(s.ar, x.ar, s.ar, e); c.ar += e;
-    two_sum(s.ai, x.ai, s.ai, e); c.ai += e;
-    two_sum(s.br, x.br, s.br, e); c.br += e;
-    two_sum(s.bi, x.bi, s.bi, e); c.bi += e;
+    if (sizeof(T) == 4) {
+      s = qadd(s, x);
+    } else {
+      accr e;
+      two_sum(s.ar, x.ar, s.ar, e); c.ar += e;
+      two_sum(s.ai, x.ai, s.ai, e); c.ai += e;
+      two_sum(s.br, x.br, s.br, e); c.br += e;
+      two_sum(s.bi, x.bi, s.bi, e); c.bi += e;
+    }
   }
   // M = U K; deriv = 2 Im-quat(M); retr = ReTr(M)
   __device__ static void project(const link &u, const acc &K, T *d, T &retr) {
-    const quat<T> M = qmul(u, K);
-    d[0] = T(2) * M.bi;
-    d[1] = T(2) * M.br;
-    d[2] = T(2) * M.ai;
-    retr = T(2) * M.ar;
+    const acc M = qmul(widen(u), K);
+    d[0] = T(2 * M.bi);
+    d[1] = T(2 * M.br);
+    d[2] = T(2 * M.ai);
+    retr = T(2 * M.ar);
   }
   __device__ static link expmul(const T *p, T eps, const link &u) {
+    if (sizeof(T) == 4) {
+      const double e0 = double(eps) * double(p[0]), e1 = double(eps) * double(p[1]),
+                   e2 = double(eps) * double(p[2]);
+      double cs, sn;
+      cos_sinc(e0 * e0 + e1 * e1 + e2 * e2, cs, sn);
+      const quat<double> n = qmul(quat<double>{cs, sn * e2, sn * e1, sn * e0}, to_d(u));
+      return link{T(n.ar), T(n.ai), T(n.br), T(n.bi)};
+    }
     return qmul(qexp(eps * p[0], eps * p[1], eps * p[2]), u);
   }
   __device__ static link restore(const link &u) { return qrestore(u); }
@@ -195,16 +238,46 @@ struct Geo {
   int str[4];  // site stride along direction mu
   int V;
   int nd;
+  // multiply-high divisors of the extents (geo_set_div): floor(n / L[mu]) =
+  // umulhi(n, mL[mu]) with mL = floor(2^32 / L) + 1, exact for n L < 2^32;
+  // L = 1 is coded as 0 (pass-through).  fdiv = 0: integer division instead.
+  uint32_t mL[4];
+  int fdiv;
 };
+
+// host: fill the divisors; exact for every n <= V when V * max L < 2^32
+inline void geo_set_div(Geo &g) {
+  uint64_t maxL = 1;
+  for (int mu = 0; mu < 4; ++mu) {
+    const uint32_t d = (uint32_t)g.L[mu];
+    g.mL[mu] = d <= 1 ? 0u : (uint32_t)((((uint64_t)1) << 32) / d + 1);
+    if ((uint64_t)d > maxL) maxL = d;
+  }
+  g.fdiv = ((uint64_t)g.V * maxL < (((uint64_t)1) << 32)) ? 1 : 0;
+}
 
 // periodic neighbour offsets of site s: up[mu] = (s+mu) - s, dn[mu] = (s-mu) - s
 template <int ND>
 __device__ __forceinline__ void neighbour_offsets(const Geo &g, int s, int *up, int *dn) {
+  int c[ND];
+  if (g.fdiv) {
+    // coordinates from the fastest direction down, one multiply-high each
+    uint32_t q = (uint32_t)s;
+#pragma unroll
+    for (int mu = ND - 1; mu > 0; --mu) {
+      const uint32_t qn = g.mL[mu] ? __umulhi(q, g.mL[mu]) : q;
+      c[mu] = (int)(q - qn * (uint32_t)g.L[mu]);
+      q = qn;
+    }
+    c[0] = (int)q;
+  } else {
+#pragma unroll
+    for (int mu = 0; mu < ND; ++mu) c[mu] = (s / g.str[mu]) % g.L[mu];
+  }
 #pragma unroll
   for (int mu = 0; mu < ND; ++mu) {
-    const int c = (s / g.str[mu]) % g.L[mu];
-    up[mu] = (c == g.L[mu] - 1) ? -(g.L[mu] - 1) * g.str[mu] : g.str[mu];
-    dn[mu] = (c == 0) ? (g.L[mu] - 1) * g.str[mu] : -g.str[mu];
+    up[mu] = (c[mu] == g.L[mu] - 1) ? -(g.L[mu] - 1) * g.str[mu] : g.str[mu];
+    dn[mu] = (c[mu] == 0) ? (g.L[mu] - 1) * g.str[mu] : -g.str[mu];
   }
 }
 

@@ -1,8 +1,10 @@
 // su2_kernels.hip -- hand-written gfx950 kernels for the pure-gauge HMC
 // molecular-dynamics step (staples, Wilson force, momentum/link updates,
 // plaquette / kinetic sums, clover energy density, layout conversion, device
-// momenta).  Every kernel is HBM-bound: no MFMA (2x2 complex products are not a
-// dense contraction).
+// momenta).  No MFMA: 2x2 complex products with a different matrix per link are
+// not a dense contraction.  The generic kernels here serve every group,
+// dimension and geometry; the SU(2) d=4 MD step and plaquette sum have
+// specialised forms in su2_md_fast.hpp.
 //
 // Field layout in HBM (one lattice):
 //   U : nd planes of V links, plane mu holds link (s, mu) at U[mu*V + s];
@@ -436,6 +438,12 @@ static int fast_shape() {
   if (v < 0) v = env_int("SU2GPU_FASTSHAPE", 0);
   return v;
 }
+// SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
+static int zs_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_ZS", 1);
+  return v;
+}
 
 template <class G, int ND>
 static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
@@ -452,11 +460,17 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     if (!make_fast_geo(L.geo.L, L.geo.V, tblock_pref() != 0, f)) return false;
     const bool acc = (L.c.flags & F_ACCUM) != 0;
     const bool site = fast_shape() == 1;
+    // +-z neighbours by lane exchange where every wave holds whole z-rows
+    const bool zs = !site && zs_pref() && (64 % f.Lz == 0);
     dim3 grid((site ? 1 : 4) * (L.geo.V / NTF)), block(NTF);
 #define SU2_FAST(W, A)                                                                          \
   if (site)                                                                                     \
     hipLaunchKernelGGL((k_md_su2_4d_site<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin, \
                        (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride);             \
+  else if (zs)                                                                                  \
+    hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, true>), grid, block, 0, st,                    \
+                       (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
+                       L.partials, L.pstride);                                                  \
   else                                                                                          \
     hipLaunchKernelGGL((k_md_su2_4d<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin,      \
                        (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride)
@@ -568,12 +582,12 @@ static void launch_plaq_g(const void *Uin, void *Uout, const Geo &g, double w_t,
       break;
   }
 }
-void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
-                 double w_s, int mu_start, int flags, double *partials, hipStream_t st) {
+int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
+                double w_s, int mu_start, int flags, double *partials, hipStream_t st) {
   FastGeo f;
   if (group == 1 && g.nd == 4 && fast_level() > 0 &&
       make_fast_geo(g.L, g.V, tblock_pref() != 0, f)) {
-    // specialised SU(2) d=4 sum: V / NTF site blocks (= site_blocks(g))
+    // specialised SU(2) d=4 sum: one thread per site, V / NTF blocks
     const dim3 grid(g.V / NTF), block(NTF);
     const bool rs = (flags & F_RESTORE) != 0;
     const int an = (flags & F_ANISO) ? 1 : 0;
@@ -588,7 +602,7 @@ void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g,
       else SU2_PLAQ(float, false);
     }
 #undef SU2_PLAQ
-    return;
+    return g.V / NTF;
   }
   if (group == 1) {
     if (prec == 0) launch_plaq_g<GSU2<double>, GSU2<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
@@ -597,6 +611,7 @@ void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g,
     if (prec == 0) launch_plaq_g<GU1<double>, GU1<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
     else launch_plaq_g<GU1<float>, GU1<double>>(Uin, Uout, g, w_t, w_s, mu_start, flags, partials, st);
   }
+  return nblk(g.V);
 }
 
 int sumsq_blocks(long n) {

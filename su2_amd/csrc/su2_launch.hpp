@@ -51,8 +51,9 @@ int fused_mode(const Geo &g);    // 0 thread per site, 1 thread per link, 2 link
 void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int nd, double eps,
                          double round_n, int flags, hipStream_t st);
 void launch_restore(int group, int prec, void *U, int n, hipStream_t st);
-void launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
-                 double w_s, int mu_start, int flags, double *partials, hipStream_t st);
+// returns the number of block partials written (one row, at most fused_blocks(g))
+int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
+                double w_s, int mu_start, int flags, double *partials, hipStream_t st);
 int sumsq_blocks(long n);
 void launch_sumsq(int prec, const void *P, long n, double *partials, hipStream_t st);
 void launch_scale(int prec, void *P, long n, double f, hipStream_t st);

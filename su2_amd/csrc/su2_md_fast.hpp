@@ -166,14 +166,6 @@ __device__ __forceinline__ void acc_d(double (&K)[4], const quat<T> &q) {
 }
 
 
-// Taylor series of cos(a) and sin(a)/a in x = a^2, Horner form, for x <= 2.5
-__device__ __forceinline__ void cos_sinc_series(double x, double &c, double &s) {
-  c = 1.0 + x * (-1.0 / 2 + x * (1.0 / 24 + x * (-1.0 / 720 + x * (1.0 / 40320 + x * (-1.0 / 3628800 +
-          x * (1.0 / 479001600 + x * (-1.0 / 87178291200.0)))))));
-  s = 1.0 + x * (-1.0 / 6 + x * (1.0 / 120 + x * (-1.0 / 5040 + x * (1.0 / 362880 + x * (-1.0 / 39916800 +
-          x * (1.0 / 6227020800.0 + x * (-1.0 / 1307674368000.0)))))));
-}
-
 // Momentum and link update of one link from its staple sum K (FP64):
 // M = U K, P <- P - ep coef 2 Im(M) rounded once, U' = exp(eu P) U.
 // Returns Re(M) and the new momentum for the block sums.
@@ -204,20 +196,7 @@ __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (
       const double e0 = c.eu * double(p0), e1 = c.eu * double(p1), e2 = c.eu * double(p2);
       const double xx = e0 * e0 + e1 * e1 + e2 * e2;
       double cs, sn;
-      // beyond the series range halve the angle k times and double back:
-      // cos 2a = 2 cos^2 a - 1, sin(2a)/(2a) = cos a sin(a)/a (never taken
-      // for MD step sizes; a^2 <= 2.5 covers |eps P| <= 1.58)
-      double xr = xx;
-      int k = 0;
-      while (xr > 2.5) {
-        xr *= 0.25;
-        ++k;
-      }
-      cos_sinc_series(xr, cs, sn);
-      for (; k > 0; --k) {
-        sn *= cs;
-        cs = 2.0 * cs * cs - 1.0;
-      }
+      cos_sinc(xx, cs, sn);  // series for a^2 <= 2.5 (|eps P| <= 1.58), else halving
       // E = (cos, sinc e2, sinc e1, sinc e0) in (Re a, Im a, Re b, Im b)
       const double Ea = cs, Eb = sn * e2, Ec = sn * e1, Ed = sn * e0;
       double na = Ea * ua - Eb * ub - Ec * uc - Ed * ud;
@@ -499,12 +478,14 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_site(const quat<T> *__restric
 }
 
 // Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) for SU(2)
-// in d = 4, one thread per site on the site layout of fast_site_s, FP64
-// whatever the storage precision:
+// in d = 4, one thread per site on the site layout of fast_site_s (V / NTF
+// blocks), FP64 whatever the storage precision:
 //   ReTr(U_mu(s) U_nu(s+mu) U_mu(s+nu)^+ U_nu(s)^+) = 2 (U_mu(s) U_nu(s+mu)) . (U_nu(s) U_mu(s+nu))
 // (4-vector dot product of two quaternion products).  RESTORE: every link is
 // reunitarised on load and the site's own restored links are written to Uout.
 // Planes with mu < mu_start are skipped; aniso weights w_t (mu = 0) / w_s.
+// (Measured at 32^4 FP32: 19 us for this shape against 43 us for one thread
+// per link with one or two planes each.)
 template <typename T, bool RESTORE>
 __global__ __launch_bounds__(NTF) void k_plaq_su2_4d(const quat<T> *__restrict__ Uin,
                                                    quat<T> *__restrict__ Uout, FastGeo g,
