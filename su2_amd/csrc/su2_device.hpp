@@ -98,6 +98,13 @@ template <typename T> __device__ __forceinline__ void two_sum(T a, T b, T &s, T 
 // formulas cos 2a = 2 cos^2 a - 1, sin(2a)/(2a) = cos a sin(a)/a.  Used where
 // the result is rounded to FP32.
 __device__ __forceinline__ void cos_sinc(double x, double &c, double &s) {
+  if (x <= 0.05) {
+    // the MD step's usual range (|eps P| <= 0.22): truncation after x^4,
+    // remainders x^5/10! and x^5/11! below 1e-13
+    c = 1.0 + x * (-1.0 / 2 + x * (1.0 / 24 + x * (-1.0 / 720 + x * (1.0 / 40320))));
+    s = 1.0 + x * (-1.0 / 6 + x * (1.0 / 120 + x * (-1.0 / 5040 + x * (1.0 / 362880))));
+    return;
+  }
   int k = 0;
   while (x > 2.5) {
     x *= 0.25;

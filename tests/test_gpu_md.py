@@ -319,6 +319,32 @@ def test_gauge_invariance_and_reversibility_32(S, oracle):
         assert abs(r["deltadeltaH"]) < 1e-7
 
 
+def test_fp32_specialised_step_32(S, oracle):
+    """The FP32 form at the benchmark size 32^4 (specialised step: t-blocked XCD
+    layout, z-lane exchange, series exp, FP64 staple sums) against the FP64 form
+    from the same links and host momenta, one OMF2(10) trajectory."""
+    dims = (32, 32, 32, 32, 4)
+    P = oracle.initnormal(1, dims, SEED + 7)
+    with S.Lattice(S.SU2, dims, 2.4) as l64:
+        l64.hotstart(SEED, 0.3)  # links within 0.3 * 2 pi of the identity
+        U = l64.get_links()
+        r64 = l64.md_trajectory(P, 6, 10)
+        l64.md_finish(True)
+        U64 = l64.get_links()
+    with S.Lattice(S.SU2, dims, 2.4, precision=S.FP32) as l32:
+        l32.set_links(U)
+        s_start = l32.gauge_action()
+        r32 = l32.md_trajectory(P, 6, 10)
+        l32.md_finish(True)
+        U32 = l32.get_links()
+        s_end = l32.gauge_action()
+    assert r32["gauge_old"] == pytest.approx(s_start, rel=1e-12)
+    assert r32["gauge_new"] == pytest.approx(s_end, rel=1e-12)
+    assert r32["kin_old"] == pytest.approx(r64["kin_old"], rel=1e-7)
+    assert np.abs(U32 - U64).max() < 1e-4
+    assert abs(r32["deltaH"] - r64["deltaH"]) < 0.05
+
+
 def test_host_program_matches_oracle(S, oracle, tmp_path):
     """The C++ host program (hmc.hpp's flow over the adapter header) on a 4^4
     lattice: output lines against the oracle's HMC program, then a restart from
