@@ -31,8 +31,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
 
-def kernel_bytes_per_link(prec: int) -> int:
-    # fused force + momentum + link update: read U, read P, write P, write U
+def sweep_bytes_per_link(prec: int):
+    """Algorithmic bytes per link of the two sweeps of an MD step, each field
+    read or written once: F = force + momentum update (read U, read P, write
+    P), G = link update (read P, read U, write U).  FP64: 80 and 88 B, FP32:
+    40 and 44 B."""
+    r = 8 if prec == 0 else 4
+    return r * (4 + 3 + 3), r * (3 + 4 + 4)
+
+
+def fused_compulsory_bytes_per_link(prec: int) -> int:
+    # the fused F + G launch keeps the new momentum in registers: read U and
+    # P, write P and U once each
     r = 8 if prec == 0 else 4
     return r * (4 + 3 + 3 + 4)
 
@@ -70,16 +80,25 @@ def run_form(S, prec, L, nsteps, steps, warmup, seed, beta):
     lat.close()
     link_updates = 4 * V * nsteps * steps
     avg_us = 1000.0 * ms_fused / max(n_fused, 1)
-    bytes_launch = kernel_bytes_per_link(prec) * 4 * V
-    achieved = bytes_launch / (avg_us * 1e-6) / 1e9
+    F, G = sweep_bytes_per_link(prec)
+    # one fused launch performs an F and a G sweep over all 4 V links
+    bytes_launch = (F + G) * 4 * V
+    compulsory = fused_compulsory_bytes_per_link(prec) * 4 * V
+    # OMF2 with n steps: 2 n + 1 F sweeps and 2 n G sweeps per trajectory
+    traj_bytes = ((2 * nsteps + 1) * F + 2 * nsteps * G) * 4 * V
+    ms_traj = 1000.0 * dt / steps
     return {
         "link_updates_per_s": link_updates / dt,
-        "ms_per_traj": 1000.0 * dt / steps,
+        "ms_per_traj": ms_traj,
         "acceptance": acc / steps,
         "mean_abs_dH": sum(abs(x) for x in dhs) / len(dhs),
         "fused_avg_us": avg_us,
         "fused_bytes_per_launch": bytes_launch,
-        "fused_achieved_GBs": achieved,
+        "fused_achieved_GBs": bytes_launch / (avg_us * 1e-6) / 1e9,
+        "fused_compulsory_bytes_per_launch": compulsory,
+        "fused_compulsory_GBs": compulsory / (avg_us * 1e-6) / 1e9,
+        "traj_bytes": traj_bytes,
+        "traj_achieved_GBs": traj_bytes / (ms_traj * 1e-3) / 1e9,
         "kernels": kinds,
     }
 
@@ -322,15 +341,18 @@ def main():
                      "unit": "GB/s", "frac": f32["fused_achieved_GBs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.L, "fp32"),
                      "kernel": kname,
-                     "avg_us": f32["fused_avg_us"], "bytes_per_launch": f32["fused_bytes_per_launch"]},
+                     "avg_us": f32["fused_avg_us"], "bytes_per_launch": f32["fused_bytes_per_launch"],
+                     "compulsory_frac": f32["fused_compulsory_GBs"] / HBM_PEAK_GBS,
+                     "trajectory_frac": f32["traj_achieved_GBs"] / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "fp32": f32,
         "fp64": dict(f64, fused_roofline_frac=f64["fused_achieved_GBs"] / HBM_PEAK_GBS,
+                     trajectory_frac=f64["traj_achieved_GBs"] / HBM_PEAK_GBS,
                      traffic=measured_traffic(args.L, "fp64")),
         "fp32_over_fp64": f32["link_updates_per_s"] / f64["link_updates_per_s"],
         "delta_dH": ddh,
         "measured_copy_GBs": copy_gbs,
-        "fused_frac_of_measured_copy_fp32": f32["fused_achieved_GBs"] / copy_gbs,
+        "fused_compulsory_frac_of_measured_copy_fp32": f32["fused_compulsory_GBs"] / copy_gbs,
         "batch": batch,
         "flow_adjoint": flow,
     }

@@ -225,10 +225,10 @@ __global__ __launch_bounds__(256) void k_batch_momenta(const BLat *__restrict__ 
     const int nm = L.group == 1 ? 3 : 1;
     T *P = (T *)ar.P + L.poff;
     for (int mu = 0; mu < L.geo.nd; ++mu) {
-      double z[4];
-      philox_normals4((uint32_t)(s * L.geo.nd + mu), (uint32_t)(tr & 0xffffffffu),
-                      (uint32_t)(tr >> 32), L.seed, z);
-      for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * L.geo.V + s] = T(z[k]);
+      T z[4];
+      philox_normals4<T>((uint32_t)(s * L.geo.nd + mu), (uint32_t)(tr & 0xffffffffu),
+                         (uint32_t)(tr >> 32), L.seed, z);
+      for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * L.geo.V + s] = z[k];
     }
   }
 }

@@ -247,12 +247,23 @@ int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row) {
   return last_launch("md_fused");
 }
 
-int reduce_row(L *h, int r, int nb) {
+// one reduce launch for every row of rl
+int reduce_rows(L *h, const su2k::RowList &rl) {
   {
     Prof pr(h, "reduce");
-    su2k::launch_reduce(row(h, r), nb, 1, h->dsums + 2 * r, h->stream);
+    su2k::launch_reduce(h->partials, (size_t)h->row_stride, rl, h->dsums, h->stream);
   }
   return last_launch("reduce");
+}
+// reduce row r (nb partials) now, or append it to *defer for a later reduce_rows
+int reduce_row(L *h, int r, int nb, su2k::RowList *defer = nullptr) {
+  su2k::RowList one;
+  su2k::RowList &rl = defer ? *defer : one;
+  if (rl.n >= su2k::RowList::MAX) return fail(SU2GPU_ESTATE, "reduce row list full");
+  rl.row[rl.n] = r;
+  rl.nb[rl.n] = nb;
+  ++rl.n;
+  return defer ? SU2GPU_OK : reduce_rows(h, rl);
 }
 int fetch_sums(L *h) {
   HC(hipMemcpyAsync(h->hsums, h->dsums, 2 * NROWS * sizeof(double), hipMemcpyDeviceToHost,
@@ -264,7 +275,8 @@ long double sum_of(const L *h, int r) {
   return (long double)h->hsums[2 * r] + (long double)h->hsums[2 * r + 1];
 }
 
-int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, int flags, int r) {
+int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, int flags, int r,
+                su2k::RowList *defer = nullptr) {
   int nb;
   {
     Prof pr(h, "plaq");
@@ -272,17 +284,17 @@ int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, i
                            h->geo, w_t, w_s, mu_start, flags, row(h, r), h->stream);
   }
   CK(last_launch("plaq"));
-  return reduce_row(h, r, nb);
+  return reduce_row(h, r, nb, defer);
 }
 
-int sumsq_launch(L *h, const void *P, int r) {
+int sumsq_launch(L *h, const void *P, int r, su2k::RowList *defer = nullptr) {
   const long n = h->nlinks * h->nm;
   {
     Prof pr(h, "sumsq");
     su2k::launch_sumsq(h->prec, P, n, row(h, r), h->stream);
   }
   CK(last_launch("sumsq"));
-  return reduce_row(h, r, su2k::sumsq_blocks(n));
+  return reduce_row(h, r, su2k::sumsq_blocks(n), defer);
 }
 
 // ------------------------------------------------------------- integrators --
@@ -947,8 +959,9 @@ int su2gpu_integrate(su2gpu_lattice *h, int integrator, size_t nsteps, double ta
 }
 
 // ---------------------------------------------------------------- md_update --
-// rows: 0..2 first fused step, 3..5 last momentum step, 8 S_old (FP32),
-//       9 S_new, 10 K_new (fallback), 12..14 / 15..17 / 18 / 19: reversibility run
+// partial rows, rbase + : 0..2 first fused step, 3..5 last momentum step,
+// 6 S_old (FP32), 7 S_new, 8 K_new (fallback); rbase 0 for the trajectory,
+// 12 for the reversibility run
 static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exponent,
                            std::initializer_list<int> protect, int rbase, int u_start,
                            long double *K_old, long double *S_old, long double *K_new,
@@ -957,8 +970,9 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
   const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
   const int pflags = h->aniso ? su2k::F_ANISO : 0;
   h->cur = u_start;
+  su2k::RowList rl;  // every sum of the trajectory is reduced in one launch at the end
   // the FP32 form takes S_old as the FP64 plaquette sum of its FP32 links
-  if (h->prec == 1) CK(plaq_launch(h, u_start, -1, w_t, w_s, 0, pflags, rbase + 6));
+  if (h->prec == 1) CK(plaq_launch(h, u_start, -1, w_t, w_s, 0, pflags, rbase + 6, &rl));
   IntegrateOut io;
   CK(run_integrator(h, kind, nsteps, tau, exponent, protect, true, rbase, &io));
   int cur = io.final_buf;
@@ -971,15 +985,16 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
     CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
                     protect.size() > 1 ? *(protect.begin() + 1) : cur},
                 &out));
-    CK(plaq_launch(h, cur, out, w_t, w_s, 0, pflags | su2k::F_RESTORE, rbase + 7));
+    CK(plaq_launch(h, cur, out, w_t, w_s, 0, pflags | su2k::F_RESTORE, rbase + 7, &rl));
     cur = out;
   } else {
-    CK(plaq_launch(h, cur, -1, w_t, w_s, 0, pflags, rbase + 7));
+    CK(plaq_launch(h, cur, -1, w_t, w_s, 0, pflags, rbase + 7, &rl));
   }
-  if (!io.last_p_only) CK(sumsq_launch(h, h->P, rbase + 8));
-  if (io.first_accum) CK(reduce_row(h, rbase + 0, nbf));
-  if (io.first_accum && h->prec == 0) CK(reduce_row(h, rbase + 1, nbf));
-  if (io.last_p_only) CK(reduce_row(h, rbase + 5, nbf));
+  if (!io.last_p_only) CK(sumsq_launch(h, h->P, rbase + 8, &rl));
+  if (io.first_accum) CK(reduce_row(h, rbase + 0, nbf, &rl));
+  if (io.first_accum && h->prec == 0) CK(reduce_row(h, rbase + 1, nbf, &rl));
+  if (io.last_p_only) CK(reduce_row(h, rbase + 5, nbf, &rl));
+  CK(reduce_rows(h, rl));
   CK(fetch_sums(h));
   *K_old = 0.5L * sum_of(h, rbase + 0);
   const long double sg = -(long double)h->beta / (long double)Nc(h);
@@ -1079,8 +1094,10 @@ int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, doub
                                 ss ? 1 : 0, row(h, 20), h->row_stride, h->stream);
   }
   CK(last_launch("energy_density"));
-  CK(reduce_row(h, 20, nbf));
-  CK(reduce_row(h, 21, nbf));
+  su2k::RowList rl;
+  CK(reduce_row(h, 20, nbf, &rl));
+  CK(reduce_row(h, 21, nbf, &rl));
+  CK(reduce_rows(h, rl));
   CK(fetch_sums(h));
   long double e = sum_of(h, 20), q = sum_of(h, 21);
   // flat-energy_density.hh:152-163

@@ -185,14 +185,16 @@ template <typename T> __global__ __launch_bounds__(NT) void k_scale(T *__restric
   if (i < n) P[i] = f * P[i];
 }
 
-// Deterministic final reduction of nval rows of nb block partials:
+// Deterministic final reduction, one block per listed row of block partials:
 // TwoSum-compensated strided accumulation, then a fixed-order LDS tree.
-// out[2*k] + out[2*k+1] is the sum of row k.
+// out[2*r] + out[2*r+1] is the sum of row r.
 constexpr int NTR = 1024;
-__global__ __launch_bounds__(NTR) void k_reduce(const double *__restrict__ partials, int nb,
+__global__ __launch_bounds__(NTR) void k_reduce(const double *__restrict__ partials,
+                                                size_t row_stride, RowList rl,
                                                 double *__restrict__ out) {
   __shared__ double hs[NTR], ls[NTR];
-  const double *row = partials + (size_t)blockIdx.x * nb;
+  const int r = rl.row[blockIdx.x], nb = rl.nb[blockIdx.x];
+  const double *row = partials + (size_t)r * row_stride;
   // four independent compensated accumulators per thread keep four loads in
   // flight; they are combined in a fixed order
   double h4[4] = {0.0, 0.0, 0.0, 0.0}, l4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -229,8 +231,8 @@ __global__ __launch_bounds__(NTR) void k_reduce(const double *__restrict__ parti
   if (threadIdx.x == 0) {
     double h, e;
     two_sum(hs[0], ls[0], h, e);
-    out[2 * blockIdx.x] = h;
-    out[2 * blockIdx.x + 1] = e;
+    out[2 * r] = h;
+    out[2 * r + 1] = e;
   }
 }
 
@@ -370,17 +372,18 @@ __global__ __launch_bounds__(NT) void k_mom_to_aos(const T *__restrict__ src, do
 }
 
 // Gaussian momenta from the counter-based Philox4x32-10 stream (device
-// throughput mode): link (s, mu) draws from counter (s*nd+mu, 2j, traj_lo,
-// traj_hi) with key = seed; Box-Muller on 53-bit uniforms.
+// throughput mode): link (s, mu) draws from counter (s*nd+mu, 0, traj_lo,
+// traj_hi) with key = seed; Box-Muller on 32-bit uniforms in the field's
+// precision.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_momenta_philox(T *__restrict__ P, int V, int nd, int nm,
                                                        uint64_t seed, uint64_t traj) {
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i >= V * nd) return;
   const int s = i / nd, mu = i - s * nd;
-  double z[4];
-  philox_normals4((uint32_t)i, (uint32_t)(traj & 0xffffffffu), (uint32_t)(traj >> 32), seed, z);
-  for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * V + s] = T(z[k]);
+  T z[4];
+  philox_normals4<T>((uint32_t)i, (uint32_t)(traj & 0xffffffffu), (uint32_t)(traj >> 32), seed, z);
+  for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * V + s] = z[k];
 }
 
 // plain streaming copy (16 B per lane per iteration): the measured HBM copy
@@ -633,8 +636,10 @@ void launch_scale(int prec, void *P, long n, double f, hipStream_t st) {
     hipLaunchKernelGGL((k_scale<float>), dim3(nblk(n)), dim3(NT), 0, st, (float *)P, (int)n, (float)f);
 }
 
-void launch_reduce(const double *partials, int nb, int nrows, double *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce, dim3(nrows), dim3(NTR), 0, st, partials, nb, out);
+void launch_reduce(const double *partials, size_t row_stride, const RowList &rl, double *out,
+                   hipStream_t st) {
+  if (rl.n > 0)
+    hipLaunchKernelGGL(k_reduce, dim3(rl.n), dim3(NTR), 0, st, partials, row_stride, rl, out);
 }
 
 template <class GS>

@@ -57,7 +57,16 @@ int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, 
 int sumsq_blocks(long n);
 void launch_sumsq(int prec, const void *P, long n, double *partials, hipStream_t st);
 void launch_scale(int prec, void *P, long n, double f, hipStream_t st);
-void launch_reduce(const double *partials, int nb, int nrows, double *out, hipStream_t st);
+// rows to reduce in one launch: row k of the partials (stride row_stride
+// doubles) holds nb[k] block partials; its sum goes to out[2 row[k] + {0,1}]
+struct RowList {
+  static constexpr int MAX = 8;
+  int n = 0;
+  int row[MAX];
+  int nb[MAX];
+};
+void launch_reduce(const double *partials, size_t row_stride, const RowList &rl, double *out,
+                   hipStream_t st);
 void launch_energy_density(int group, int prec, const void *U, const Geo &g, int clover,
                            int mu_start, double *partials, int pstride, hipStream_t st);
 void launch_links_from_aos(int group, int prec, const double *src, int stride, int offset,
@@ -108,18 +117,26 @@ __host__ __device__ inline double philox_u01(uint32_t hi, uint32_t lo) {
   return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-// four N(0,1) draws for counter (i, 0|1, t0, t1) and key = seed (Box-Muller;
-// the angle 2 pi u2 through sincospi, which needs no range reduction)
+// Four N(0,1) draws for counter (i, 0, t0, t1) and key = seed: one
+// Philox4x32-10 block, Box-Muller on the word pairs (w0, w1) and (w2, w3) with
+// radius uniform (w0 + 1) 2^-32 in (0, 1] and angle 2 pi w1 2^-32 (sincospi
+// needs no range reduction).  T = double: exact uniforms, FP64 log and
+// sincospi.  T = float: the same uniforms rounded to FP32 and FP32 log and
+// sincospi, so both forms draw the same momenta to FP32 accuracy (the radius
+// differs by at most ~3e-4 in the 2^-24-probability corner u -> 1, r -> 0).
+template <typename T>
 __device__ inline void philox_normals4(uint32_t i, uint32_t t0, uint32_t t1, uint64_t seed,
-                                       double z[4]) {
+                                       T z[4]) {
+  uint32_t c[4] = {i, 0u, t0, t1};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    uint32_t c[4] = {i, (uint32_t)h, t0, t1};
-    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    const double u1 = philox_u01(c[0], c[1]), u2 = philox_u01(c[2], c[3]);
-    const double r = sqrt(-2.0 * log(u1));
-    double sn, cs;
-    sincospi(2.0 * u2, &sn, &cs);
+    const T u1 = ((T)c[2 * h] + T(1)) * T(0x1p-32);
+    const T u2 = (T)c[2 * h + 1] * T(0x1p-32);
+    const T r = sqrt(T(-2) * log(u1));
+    T sn, cs;
+    if constexpr (sizeof(T) == 4) sincospif(2.0f * u2, &sn, &cs);
+    else sincospi(T(2) * u2, &sn, &cs);
     z[2 * h] = r * cs;
     z[2 * h + 1] = r * sn;
   }

@@ -18,8 +18,9 @@
 //   * the force projection and the momentum update run in FP64 on the FP32
 //     operands and round the new momentum once;
 //   * FP32: exp(eps P) = (cos a, sin(a)/a eps P) from the Taylor series of cos
-//     and sin(x)/x in x = a^2 (Horner, truncated at x^7: remainder below
-//     7e-11 for a^2 <= 2.5; larger angles by halving and double-angle formulas);
+//     and sin(x)/x in x = a^2 (Horner; for a^2 <= 0.05 truncated after x^3,
+//     remainder below 2e-10; else truncated at x^7, remainder below 7e-11 for a^2 <= 2.5, larger angles by
+//     halving and double-angle formulas);
 //   * FP32: U' = E U in FP64 and reunitarised in FP64 before the single
 //     rounding to FP32 (unbiased; see qrestore<float>).
 // FP64 links keep the generic exp (sincos) and the reference's order of
@@ -31,6 +32,26 @@
 namespace su2k {
 
 constexpr int NTF = 256;  // threads per workgroup of the specialised step
+
+// cos a and sin(a)/a for x = a^2.  The MD step's range x <= 0.05 (|eps P| <=
+// 0.22): series truncated after x^3 (remainders x^4/8! and x^4/9! below 2e-10,
+// far below the FP32 rounding of the updated link); larger x: cos_sinc.
+__device__ __forceinline__ void cos_sinc_step(double x, double &c, double &s) {
+  if (x <= 0.05) {
+    // c = 1 - x/2 + x^2/24 - x^3/720, s = 1 - x/6 + x^2/120 - x^3/5040, in a
+    // factored Horner form whose non-inline constants are all multiplicands
+    // (scalar-register operands, no vector copies of FP64 literals)
+    double t = fma(x, -1.0 / 30, 1.0);  // 1 - x/30
+    t *= 1.0 / 24;                      // 1/24 - x/720
+    t = fma(x, t, -0.5);
+    c = fma(x, t, 1.0);
+    double w = fma(x, -1.0 / 42, 1.0);  // 1 - x/42
+    w = fma(x * (-1.0 / 20), w, 1.0);   // 1 - x/20 + x^2/840
+    s = fma(x * (-1.0 / 6), w, 1.0);    // 1 - x/6 + x^2/120 - x^3/5040
+    return;
+  }
+  cos_sinc(x, c, s);
+}
 
 // Launch layout of the specialised step.  nb = 4 V / NT blocks; the hardware
 // dispatches block b to XCD b % 8.  With tblock, XCD k owns the t-slices
@@ -196,7 +217,7 @@ __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (
       const double e0 = c.eu * double(p0), e1 = c.eu * double(p1), e2 = c.eu * double(p2);
       const double xx = e0 * e0 + e1 * e1 + e2 * e2;
       double cs, sn;
-      cos_sinc(xx, cs, sn);  // series for a^2 <= 2.5 (|eps P| <= 1.58), else halving
+      cos_sinc_step(xx, cs, sn);
       // E = (cos, sinc e2, sinc e1, sinc e0) in (Re a, Im a, Re b, Im b)
       const double Ea = cs, Eb = sn * e2, Ec = sn * e1, Ed = sn * e0;
       double na = Ea * ua - Eb * ub - Ec * uc - Ed * ud;
@@ -206,7 +227,7 @@ __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (
       if (c.flags & F_RESTORE) {
         // 1/sqrt(1 + d) = 1 - d/2 + 3 d^2/8 - ..., remainder below 3e-13 for
         // |d| < 1e-4 (an FP32 link times the series exp has |d| ~ 1e-7)
-        const double d = (na * na + nb * nb + nc * nc + nd * nd) - 1.0;
+        const double d = fma(na, na, fma(nb, nb, fma(nc, nc, fma(nd, nd, -1.0))));
         const double r = (fabs(d) < 1e-4) ? 1.0 + d * (-0.5 + 0.375 * d) : 1.0 / sqrt(1.0 + d);
         na *= r;
         nb *= r;
@@ -262,7 +283,9 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
   T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
   const quat<T> u = ldo(Um, (uint32_t)s);
   const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
-  double K[4] = {0.0, 0.0, 0.0, 0.0};
+  // -0.0 is the identity of FP64 addition (+0.0 is not: +0.0 + -0.0 = +0.0),
+  // so the first staple's addition folds away
+  double K[4] = {-0.0, -0.0, -0.0, -0.0};
   if constexpr (MU >= 0) {
     quat<T> A[3], B[3], C[3], D[3], E[3], F[3];
     // ZS: the +-z neighbours of the wave's sites sit in the neighbouring lanes
@@ -414,7 +437,7 @@ __device__ __forceinline__ void md_su2_site_body(const quat<T> *__restrict__ Uin
   for (int m = 0; m < 4; ++m) own[m] = ldo(Uin + (size_t)m * V, (uint32_t)s);
   double K[4][4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) K[m][0] = K[m][1] = K[m][2] = K[m][3] = 0.0;
+  for (int m = 0; m < 4; ++m) K[m][0] = K[m][1] = K[m][2] = K[m][3] = -0.0;  // additive identity
 #pragma unroll
   for (int mu = 0; mu < 3; ++mu) {
 #pragma unroll

@@ -299,6 +299,37 @@ def test_device_momenta_statistics(S):
         assert abs(2 * K / n - 1.0) < 5 * math.sqrt(2.0 / n)
 
 
+def test_device_momenta_distribution(S):
+    """Device Philox momenta (one Philox4x32-10 block per link, Box-Muller on
+    32-bit uniforms): N(0,1) moments and a Kolmogorov-Smirnov test on 786k
+    draws, independence of the components, and the FP32 form drawing the same
+    momenta as the FP64 form to FP32 accuracy.  A leapfrog step of length
+    1e-30 leaves the drawn momenta unchanged, so get_momenta returns them."""
+    from scipy import stats
+    dims = (16, 16, 16, 16, 4)
+    draws = {}
+    for prec in (S.FP64, S.FP32):
+        with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
+            lat.hotstart(SEED)
+            lat.md_update(SEED + 5, S.LEAPFROG, 1, tau=1e-30, momenta=S.MOMENTA_DEVICE)
+            draws[prec] = lat.get_momenta().reshape(-1, 3)  # (link, component)
+    z = draws[S.FP64].ravel()
+    n = z.size
+    assert abs(z.mean()) < 5 / math.sqrt(n)
+    assert abs(z.var() - 1.0) < 5 * math.sqrt(2.0 / n)
+    assert abs((z ** 4).mean() - 3.0) < 5 * math.sqrt(96.0 / n)
+    assert stats.kstest(z, "norm").pvalue > 1e-4
+    # the three components of one link (two Box-Muller pairs of one block)
+    P = draws[S.FP64]
+    nl = P.shape[0]
+    assert np.abs(np.corrcoef(P.T) - np.eye(3)).max() < 5 / math.sqrt(nl)
+    # neighbouring links (consecutive Philox counters) are independent
+    assert abs(np.corrcoef(P[0::2, 0], P[1::2, 0])[0, 1]) < 5 / math.sqrt(nl / 2)
+    d = np.abs(draws[S.FP32] - draws[S.FP64]).ravel()
+    assert d.max() < 1e-3
+    assert (d < 2e-6).mean() > 0.9999
+
+
 def test_gauge_invariance_and_reversibility_32(S, oracle):
     """Size-independent properties at the benchmark size 32^4."""
     dims = (32, 32, 32, 32, 4)
