@@ -376,6 +376,63 @@ def test_fp32_specialised_step_32(S, oracle):
     assert abs(r32["deltaH"] - r64["deltaH"]) < 0.05
 
 
+@pytest.mark.parametrize("dims", [
+    (8, 8, 16, 24, 4),   # t-blocked, Lz = 24: no z-lane exchange
+    (6, 8, 8, 16, 4),    # Lt % 8 != 0: XCD chunks without t-blocking, z-lane exchange
+    (12, 4, 8, 24, 4),   # neither
+    (8, 2, 4, 32, 4),    # Lx = 2, Ly = 4: x and y neighbours wrap within a block
+])
+def test_fp32_specialised_step_geometries(S, oracle, dims):
+    """The specialised FP32 step on the launch layouts other than 32^4's (every
+    3-volume here is a multiple of 256, so the FP32 form takes it) against the
+    FP64 form from the same links and host momenta, one OMF2(5) trajectory."""
+    P = oracle.initnormal(1, dims, SEED + 11)
+    U = oracle.hotstart(1, dims, SEED + 11, 0.3)
+    with S.Lattice(S.SU2, dims, 2.4) as l64:
+        l64.set_links(U)
+        r64 = l64.md_trajectory(P, 6, 5)
+        l64.md_finish(True)
+        U64 = l64.get_links()
+    Uw, _ = oracle.integrate(1, dims, U, P, 2.4, 6, 5)
+    np.testing.assert_allclose(U64, Uw, atol=1e-11)
+    with S.Lattice(S.SU2, dims, 2.4, precision=S.FP32) as l32:
+        l32.set_links(U)
+        s_start = l32.gauge_action()
+        r32 = l32.md_trajectory(P, 6, 5)
+        l32.md_finish(True)
+        U32 = l32.get_links()
+        s_end = l32.gauge_action()
+    assert r32["gauge_old"] == pytest.approx(s_start, rel=1e-12)
+    assert r32["gauge_new"] == pytest.approx(s_end, rel=1e-12)
+    assert np.abs(U32 - U64).max() < 2e-5
+    assert abs(r32["deltaH"] - r64["deltaH"]) < 2e-3
+    q = U32.reshape(-1, 4)
+    assert np.abs((q ** 2).sum(1) - 1.0).max() < 5e-7
+
+
+def test_fp64_trajectory_parity_32(S, oracle):
+    """The plain FP64 form at the benchmark size: one OMF2(20) trajectory on
+    32^4 at beta = 2.4 from the seeded hot start with the reference's host
+    momenta stream (initnormal, mt19937), against the oracle: max per-link
+    deviation <= 1e-10, dH and the plaquette to 1e-9 relative."""
+    dims = (32, 32, 32, 32, 4)
+    beta = 2.4
+    U = oracle.hotstart(1, dims, SEED, 1.0)
+    P = oracle.initnormal(1, dims, SEED + 1)
+    Uw, Pw = oracle.integrate(1, dims, U, P, beta, 6, 20)
+    h_old = oracle.action(1, dims, U, beta, compensated=True) + oracle.kinetic(1, dims, P, compensated=True)
+    h_new = oracle.action(1, dims, Uw, beta, compensated=True) + oracle.kinetic(1, dims, Pw, compensated=True)
+    with S.Lattice(S.SU2, dims, beta) as lat:
+        lat.set_links(U)
+        r = lat.md_trajectory(P, 6, 20)
+        lat.md_finish(True)
+        Ug = lat.get_links()
+        plaq = lat.gauge_energy()
+    assert np.abs(Ug - Uw).max() <= 1e-10
+    assert r["deltaH"] == pytest.approx(h_new - h_old, rel=1e-9, abs=1e-8)
+    assert plaq == pytest.approx(oracle.retr_sum_plaq(1, dims, Uw, compensated=True), rel=1e-9)
+
+
 def test_host_program_matches_oracle(S, oracle, tmp_path):
     """The C++ host program (hmc.hpp's flow over the adapter header) on a 4^4
     lattice: output lines against the oracle's HMC program, then a restart from
