@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <random>
@@ -97,6 +98,8 @@ struct su2gpu_lattice {
     void *W1 = nullptr, *W2 = nullptr;  // stage states of one step
     void *Zs[2] = {nullptr, nullptr};   // Z after stages 0 and 1
     void *lamW = nullptr, *lamZ = nullptr, *Gam = nullptr;
+    void *R = nullptr;                  // scratch of the adjoint's plaquette pass
+    std::vector<void *> stU, stZ;       // stage states of a segment: W1, W2 and Z0', Z1', Z2' per step
   } flow;
 };
 
@@ -676,7 +679,9 @@ int su2gpu_destroy(su2gpu_lattice *h) {
     auto &f = h->flow;
     for (void *p : f.ckpts) hipFree(p);
     for (void *p : f.seg) hipFree(p);
-    for (void *p : {f.W[0], f.W[1], f.Z, f.W1, f.W2, f.Zs[0], f.Zs[1], f.lamW, f.lamZ, f.Gam})
+    for (void *p : f.stU) hipFree(p);
+    for (void *p : f.stZ) hipFree(p);
+    for (void *p : {f.W[0], f.W[1], f.Z, f.W1, f.W2, f.Zs[0], f.Zs[1], f.lamW, f.lamZ, f.Gam, f.R})
       if (p) hipFree(p);
   }
   for (int i = 0; i < NBUF; ++i)
@@ -1319,6 +1324,7 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
   CK(flow_alloc(h, &f.Zs[1], h->P_bytes));
   CK(flow_alloc(h, &f.lamW, h->P_bytes));
   CK(flow_alloc(h, &f.lamZ, h->P_bytes));
+  CK(flow_alloc(h, &f.R, (size_t)h->V * 36 * h->real_bytes));
   if (f.seg.size() < f.ckpt) {
     size_t old = f.seg.size();
     f.seg.resize(f.ckpt, nullptr);
@@ -1343,24 +1349,71 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
   }
   double b[3], c[3];
   rk_coef(h, f.eps, b, c);
+  // The stage states of a whole segment (per step W1, W2 and Z0', Z1', Z2') are
+  // kept from its forward recomputation when they fit in half of the free
+  // device memory; otherwise every step recomputes its stages from W_k (the
+  // same kernels on the same inputs: identical results, one more forward pass).
+  // SU2GPU_FLOW_RECOMPUTE=1 forces the recomputation.
+  const char *rc = std::getenv("SU2GPU_FLOW_RECOMPUTE");
+  bool keep = !(rc && std::atoi(rc) != 0);
+  if (keep && f.stU.size() < 2 * f.ckpt) {
+    size_t fr = 0, tot = 0;
+    HC(hipMemGetInfo(&fr, &tot));
+    const size_t need = (2 * f.ckpt - f.stU.size()) * h->U_bytes +
+                        (3 * f.ckpt - f.stZ.size()) * h->P_bytes;
+    keep = need < fr / 2;
+    if (keep) {
+      while (f.stU.size() < 2 * f.ckpt) {
+        f.stU.push_back(nullptr);
+        HC(hipMalloc(&f.stU.back(), h->U_bytes));
+      }
+      while (f.stZ.size() < 3 * f.ckpt) {
+        f.stZ.push_back(nullptr);
+        HC(hipMalloc(&f.stZ.back(), h->P_bytes));
+      }
+    }
+  }
   const size_t nseg = f.ckpts.size();
   for (size_t sg = nseg; sg-- > 0;) {
     const size_t k0 = sg * f.ckpt, k1 = std::min(f.nsteps, k0 + f.ckpt);
     // recompute W_k, k0 <= k < k1, from the checkpoint
     HC(hipMemcpyAsync(f.seg[0], f.ckpts[sg], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
-    for (size_t k = k0; k + 1 < k1; ++k)
-      CK(rk_step(h, f.seg[k - k0], f.seg[k - k0 + 1], f.W1, f.Z, f.eps));
+    if (keep) {
+      // stage by stage, keeping W1, W2 and the three Z'
+      for (size_t k = k0; k < k1; ++k) {
+        const size_t j = k - k0;
+        CK(rk_stage(h, f.seg[j], f.stU[2 * j], f.Z, b[0], c[0], true));
+        HC(hipMemcpyAsync(f.stZ[3 * j], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+        CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.Z, b[1], c[1], false));
+        HC(hipMemcpyAsync(f.stZ[3 * j + 1], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+        CK(rk_stage(h, f.stU[2 * j + 1], k + 1 < k1 ? f.seg[j + 1] : nullptr, f.Z, b[2], c[2],
+                    false));
+        HC(hipMemcpyAsync(f.stZ[3 * j + 2], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+      }
+    } else {
+      for (size_t k = k0; k + 1 < k1; ++k)
+        CK(rk_step(h, f.seg[k - k0], f.seg[k - k0 + 1], f.W1, f.Z, f.eps));
+    }
     for (size_t k = k1; k-- > k0;) {
-      const void *W0 = f.seg[k - k0];
-      // stage states: Z0' = b0 D(W0), W1 = exp(c0 Z0') W0, Z1' = Z0' + b1 D(W1),
-      // W2 = exp(c1 Z1') W1, Z2' = Z1' + b2 D(W2)
-      CK(rk_stage(h, W0, f.W1, f.Z, b[0], c[0], true));
-      HC(hipMemcpyAsync(f.Zs[0], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
-      CK(rk_stage(h, f.W1, f.W2, f.Z, b[1], c[1], false));
-      HC(hipMemcpyAsync(f.Zs[1], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
-      CK(rk_stage(h, f.W2, nullptr, f.Z, b[2], c[2], false));
+      const size_t j = k - k0;
+      const void *W0 = f.seg[j];
       const void *Wst[3] = {W0, f.W1, f.W2};
       const void *Zst[3] = {f.Zs[0], f.Zs[1], f.Z};
+      if (keep) {
+        Wst[1] = f.stU[2 * j];
+        Wst[2] = f.stU[2 * j + 1];
+        Zst[0] = f.stZ[3 * j];
+        Zst[1] = f.stZ[3 * j + 1];
+        Zst[2] = f.stZ[3 * j + 2];
+      } else {
+        // stage states: Z0' = b0 D(W0), W1 = exp(c0 Z0') W0, Z1' = Z0' + b1 D(W1),
+        // W2 = exp(c1 Z1') W1, Z2' = Z1' + b2 D(W2)
+        CK(rk_stage(h, W0, f.W1, f.Z, b[0], c[0], true));
+        HC(hipMemcpyAsync(f.Zs[0], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+        CK(rk_stage(h, f.W1, f.W2, f.Z, b[1], c[1], false));
+        HC(hipMemcpyAsync(f.Zs[1], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+        CK(rk_stage(h, f.W2, nullptr, f.Z, b[2], c[2], false));
+      }
       for (int st = 2; st >= 0; --st) {
         {
           Prof pr(h, "flow_stage_adj");
@@ -1370,7 +1423,7 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
         CK(last_launch("flow_stage_adj"));
         {
           Prof pr(h, "flow_vjp");
-          su2k::launch_flow_vjp(h->prec, Wst[st], f.lamZ, f.lamW, h->geo, b[st], w_t, w_s,
+          su2k::launch_flow_vjp(h->prec, Wst[st], f.lamZ, f.lamW, f.R, h->geo, b[st], w_t, w_s,
                                 h->stream);
         }
         CK(last_launch("flow_vjp"));

@@ -10,10 +10,12 @@
 // (the force without beta/Nc).  Its adjoint:
 //   lambda_Z' += c J(cZ')^T lambda_W'      J(A) = sum ad_A^k/(k+1)!, J^T = J(-A)
 //   lambda_W   = Ad(exp(cZ'))^T lambda_W'  (= Ad(exp(-cZ')))
-//   lambda_W  += b (dD/dW)^T lambda_Z'     (Hessian-vector product, k_flow_vjp)
+//   lambda_W  += b (dD/dW)^T lambda_Z'     (Hessian-vector product, k_flow_vjp_*)
 //   lambda_Z   = a lambda_Z'
 //
-// (dD/dW)^T lambda is gathered per link from its 2(d-1) plaquettes.  For the
+// (dD/dW)^T lambda collects, per link, one term from each of its 2(d-1)
+// plaquettes; every plaquette is evaluated once and gives the terms of its four
+// links (k_flow_vjp_plaq, k_flow_vjp_gather).  For the
 // plaquette P = a b c^+ d^+ (a = U_mu(x), b = U_nu(x+mu), c = U_mu(x+nu),
 // d = U_nu(x)) sum_m lambda_m . D_m restricted to P equals -2 Re(P Omega) with
 // Omega = L_a + a L_b a^+ - d L_c d^+ - L_d (L = pure quaternion of lambda), and
@@ -107,34 +109,31 @@ __global__ __launch_bounds__(NTF) void k_flow_stage_adj(const T *__restrict__ Zs
   lamW[(mu * 3 + 2) * V + s] = yw.z;
 }
 
-// role-specific gradient of -Re(P Omega) for the HVP insertion pattern
+// the four role gradients of -Re(P Omega) for the HVP insertion pattern of one
+// plaquette (a, b, c, d and their cotangents La..Ld as in the header); P, Omega,
+// P Omega and Omega P are shared by the roles
+template <typename T> struct roles4 {
+  v3<T> ga, gb, gc, gd;
+};
 template <typename T>
-__device__ __forceinline__ v3<T> hvp_role(int role, const quat<T> &a, const quat<T> &b,
-                                          const quat<T> &c, const quat<T> &d, const v3<T> &La,
-                                          const v3<T> &Lb, const v3<T> &Lc, const v3<T> &Ld) {
+__device__ __forceinline__ roles4<T> hvp_roles(const quat<T> &a, const quat<T> &b, const quat<T> &c,
+                                               const quat<T> &d, const v3<T> &La, const v3<T> &Lb,
+                                               const v3<T> &Lc, const v3<T> &Ld) {
   const quat<T> P = qmul(qmul(qmul(a, b), qdag(c)), qdag(d));
   const quat<T> Tb = conj_by(a, vq(Lb));
   const quat<T> Tc = qscale(T(-1), conj_by(d, vq(Lc)));
   const quat<T> Om = qsub(qadd(qadd(vq(La), Tb), Tc), vq(Ld));
-  switch (role) {
-    case 0: {  // a
-      v3<T> r = im(qmul(P, Om));
-      vadd(r, im(comm(Tb, P)));
-      return r;
-    }
-    case 1:  // b
-      return im(qmul(qmul(qmul(qdag(a), P), Om), a));
-    case 2: {  // c
-      v3<T> r = im(qmul(qmul(qmul(qdag(d), Om), P), d));
-      return v3<T>{-r.x, -r.y, -r.z};
-    }
-    default: {  // d
-      v3<T> r = im(qmul(Om, P));
-      r = v3<T>{-r.x, -r.y, -r.z};
-      vadd(r, im(comm(Tc, P)));
-      return r;
-    }
-  }
+  const quat<T> PO = qmul(P, Om), OP = qmul(Om, P);
+  roles4<T> r;
+  r.ga = im(PO);  // a: Im(P Om) + Im([Tb, P])
+  vadd(r.ga, im(comm(Tb, P)));
+  r.gb = im(qmul(qmul(qdag(a), PO), a));  // b: Im(a^+ P Om a)
+  const v3<T> c3 = im(qmul(qmul(qdag(d), OP), d));  // c: -Im(d^+ Om P d)
+  r.gc = v3<T>{-c3.x, -c3.y, -c3.z};
+  const v3<T> d3 = im(OP);  // d: -Im(Om P) + Im([Tc, P])
+  r.gd = v3<T>{-d3.x, -d3.y, -d3.z};
+  vadd(r.gd, im(comm(Tc, P)));
+  return r;
 }
 
 // role-specific gradient of -Re(P Omega_E) for the clover insertions
@@ -179,45 +178,84 @@ __device__ __forceinline__ int plane_id(int mu, int nu) {
   return mu == 0 ? nu - 1 : (mu == 1 ? nu + 1 : 5);
 }
 
-// lamW += bcoef * (dD/dW)^T lamZ   (per site, all link directions)
+// lamW += bcoef * (dD/dW)^T lamZ in two passes.  Pass 1, one thread per site x:
+// each plaquette P_{mu nu}(x), mu < nu, is evaluated once and gives its four
+// role gradients; roles a and d belong to the site's own links (x, mu) and
+// (x, nu) and are summed in registers (one read-modify-write of lamW per link);
+// roles b and c belong to (x+mu, nu) and (x+nu, mu) and go to the scratch rows
+// R[(p*2 + r)*3 + k][x] (p = plane_id, r = 0: b, 1: c), which pass 2 gathers.
+// Every plaquette is evaluated once instead of once per link (four times).
 template <typename T, int ND>
-__global__ __launch_bounds__(NTF) void k_flow_vjp(const quat<T> *__restrict__ U,
-                                                  const T *__restrict__ lamZ, T *__restrict__ lamW,
-                                                  Geo geo, double bcoef, double w_t, double w_s) {
+__global__ __launch_bounds__(NTF) void k_flow_vjp_plaq(const quat<T> *__restrict__ U,
+                                                       const T *__restrict__ lamZ,
+                                                       T *__restrict__ lamW, T *__restrict__ R,
+                                                       Geo geo, double bcoef, double w_t,
+                                                       double w_s) {
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int s = lb * NTF + threadIdx.x;
   const int V = geo.V;
   if (s >= V) return;
   int up[ND], dn[ND];
   neighbour_offsets<ND>(geo, s, up, dn);
-  auto Ul = [&](int d, int site) { return U[d * V + site]; };
-  auto Ll = [&](int d, int site) { return ld3(lamZ, d, V, site); };
+  v3<T> acc[ND];
+#pragma unroll
+  for (int m = 0; m < ND; ++m) acc[m] = v3<T>{0, 0, 0};
+#pragma unroll
+  for (int mu = 0; mu < ND; ++mu) {
+#pragma unroll
+    for (int nu = mu + 1; nu < ND; ++nu) {
+      const T w = T(mu == 0 ? w_t : w_s);
+      const roles4<T> r = hvp_roles<T>(
+          U[mu * V + s], U[nu * V + s + up[mu]], U[mu * V + s + up[nu]], U[nu * V + s],
+          ld3(lamZ, mu, V, s), ld3(lamZ, nu, V, s + up[mu]), ld3(lamZ, mu, V, s + up[nu]),
+          ld3(lamZ, nu, V, s));
+      vadd(acc[mu], r.ga, w);
+      vadd(acc[nu], r.gd, w);
+      const int p = plane_id(mu, nu);
+      T *Rb = R + (size_t)(p * 2 + 0) * 3 * V, *Rc = R + (size_t)(p * 2 + 1) * 3 * V;
+      Rb[0 * V + s] = w * r.gb.x;
+      Rb[1 * V + s] = w * r.gb.y;
+      Rb[2 * V + s] = w * r.gb.z;
+      Rc[0 * V + s] = w * r.gc.x;
+      Rc[1 * V + s] = w * r.gc.y;
+      Rc[2 * V + s] = w * r.gc.z;
+    }
+  }
+  const T f = T(2.0 * bcoef);
+#pragma unroll
+  for (int rho = 0; rho < ND; ++rho) {
+    lamW[(rho * 3 + 0) * V + s] += f * acc[rho].x;
+    lamW[(rho * 3 + 1) * V + s] += f * acc[rho].y;
+    lamW[(rho * 3 + 2) * V + s] += f * acc[rho].z;
+  }
+}
+
+// pass 2, one thread per site x, for each link (x, rho) and sigma != rho: the
+// role c of P_{rho sigma}(x - sigma) (rho < sigma) or the role b of
+// P_{sigma rho}(x - sigma) (rho > sigma), added in the fixed sigma order, so the
+// result is deterministic
+template <typename T, int ND>
+__global__ __launch_bounds__(NTF) void k_flow_vjp_gather(const T *__restrict__ R,
+                                                         T *__restrict__ lamW, Geo geo,
+                                                         double bcoef) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NTF + threadIdx.x;
+  const int V = geo.V;
+  if (s >= V) return;
+  int up[ND], dn[ND];
+  neighbour_offsets<ND>(geo, s, up, dn);
+  const T f = T(2.0 * bcoef);
 #pragma unroll
   for (int rho = 0; rho < ND; ++rho) {
     v3<T> g{0, 0, 0};
 #pragma unroll
     for (int sg = 0; sg < ND; ++sg) {
       if (sg == rho) continue;
-      const T w = T((rho == 0 || sg == 0) ? w_t : w_s);
-      const int sm = s + dn[sg];  // s - sigma
-      v3<T> gi, gii;
-      if (rho < sg) {
-        // x = s (mu = rho, nu = sg): role a;  x = s - sg: role c
-        gi = hvp_role<T>(0, Ul(rho, s), Ul(sg, s + up[rho]), Ul(rho, s + up[sg]), Ul(sg, s),
-                         Ll(rho, s), Ll(sg, s + up[rho]), Ll(rho, s + up[sg]), Ll(sg, s));
-        gii = hvp_role<T>(2, Ul(rho, sm), Ul(sg, sm + up[rho]), Ul(rho, s), Ul(sg, sm), Ll(rho, sm),
-                          Ll(sg, sm + up[rho]), Ll(rho, s), Ll(sg, sm));
-      } else {
-        // x = s (mu = sg, nu = rho): role d;  x = s - sg: role b
-        gi = hvp_role<T>(3, Ul(sg, s), Ul(rho, s + up[sg]), Ul(sg, s + up[rho]), Ul(rho, s),
-                         Ll(sg, s), Ll(rho, s + up[sg]), Ll(sg, s + up[rho]), Ll(rho, s));
-        gii = hvp_role<T>(1, Ul(sg, sm), Ul(rho, s), Ul(sg, sm + up[rho]), Ul(rho, sm), Ll(sg, sm),
-                          Ll(rho, s), Ll(sg, sm + up[rho]), Ll(rho, sm));
-      }
-      vadd(g, gi, w);
-      vadd(g, gii, w);
+      const int p = rho < sg ? plane_id(rho, sg) : plane_id(sg, rho);
+      const T *Rr = R + (size_t)(p * 2 + (rho < sg ? 1 : 0)) * 3 * V;
+      const int sm = s + dn[sg];
+      vadd(g, v3<T>{Rr[sm], Rr[V + sm], Rr[2 * V + sm]});
     }
-    const T f = T(2.0 * bcoef);
     lamW[(rho * 3 + 0) * V + s] += f * g.x;
     lamW[(rho * 3 + 1) * V + s] += f * g.y;
     lamW[(rho * 3 + 2) * V + s] += f * g.z;
@@ -314,20 +352,27 @@ void launch_flow_stage_adj(int prec, const void *Zs, void *lamW, void *lamZ, int
                        (float *)lamW, (float *)lamZ, V, nd, c, zero_lamZ);
 }
 
+template <typename T, int ND>
+static void vjp_nd(const void *U, const void *lamZ, void *lamW, void *R, const Geo &g, double b,
+                   double w_t, double w_s, hipStream_t st) {
+  hipLaunchKernelGGL((k_flow_vjp_plaq<T, ND>), dim3(nblkf(g.V)), dim3(NTF), 0, st,
+                     (const quat<T> *)U, (const T *)lamZ, (T *)lamW, (T *)R, g, b, w_t, w_s);
+  hipLaunchKernelGGL((k_flow_vjp_gather<T, ND>), dim3(nblkf(g.V)), dim3(NTF), 0, st,
+                     (const T *)R, (T *)lamW, g, b);
+}
 template <typename T>
-static void vjp_t(const void *U, const void *lamZ, void *lamW, const Geo &g, double b, double w_t,
-                  double w_s, hipStream_t st) {
-  dim3 grid(nblkf(g.V));
+static void vjp_t(const void *U, const void *lamZ, void *lamW, void *R, const Geo &g, double b,
+                  double w_t, double w_s, hipStream_t st) {
   switch (g.nd) {
-    case 2: hipLaunchKernelGGL((k_flow_vjp<T, 2>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)lamZ, (T *)lamW, g, b, w_t, w_s); break;
-    case 3: hipLaunchKernelGGL((k_flow_vjp<T, 3>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)lamZ, (T *)lamW, g, b, w_t, w_s); break;
-    default: hipLaunchKernelGGL((k_flow_vjp<T, 4>), grid, dim3(NTF), 0, st, (const quat<T> *)U, (const T *)lamZ, (T *)lamW, g, b, w_t, w_s); break;
+    case 2: vjp_nd<T, 2>(U, lamZ, lamW, R, g, b, w_t, w_s, st); break;
+    case 3: vjp_nd<T, 3>(U, lamZ, lamW, R, g, b, w_t, w_s, st); break;
+    default: vjp_nd<T, 4>(U, lamZ, lamW, R, g, b, w_t, w_s, st); break;
   }
 }
-void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, const Geo &g, double b,
-                     double w_t, double w_s, hipStream_t st) {
-  if (prec == 0) vjp_t<double>(U, lamZ, lamW, g, b, w_t, w_s, st);
-  else vjp_t<float>(U, lamZ, lamW, g, b, w_t, w_s, st);
+void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, void *R, const Geo &g,
+                     double b, double w_t, double w_s, hipStream_t st) {
+  if (prec == 0) vjp_t<double>(U, lamZ, lamW, R, g, b, w_t, w_s, st);
+  else vjp_t<float>(U, lamZ, lamW, R, g, b, w_t, w_s, st);
 }
 
 template <typename T>

@@ -85,7 +85,8 @@ void launch_copy16(const void *src, void *dst, long n16, hipStream_t st);
 // gradient-flow adjoint (su2_flow.hip)
 void launch_flow_stage_adj(int prec, const void *Zs, void *lamW, void *lamZ, int V, int nd,
                            double c, int zero_lamZ, hipStream_t st);
-void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, const Geo &g,
+// R: scratch of 36 V reals (the b and c role gradients of the 6 planes)
+void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, void *R, const Geo &g,
                      double b, double w_t, double w_s, hipStream_t st);
 void launch_clover_grad(int prec, const void *U, void *Gam, void *lamW, const Geo &g,
                         double scale, hipStream_t st);

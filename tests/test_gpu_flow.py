@@ -89,3 +89,41 @@ def test_fp32_gradient_close_to_fp64(S, oracle):
             grads.append(lat.flow_backward())
     rel = np.abs(grads[1] - grads[0]).max() / np.abs(grads[0]).max()
     assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("obs", [0, 1])
+@pytest.mark.parametrize("dims,xi", [((4, 5, 1, 3, 3), 1.0), ((4, 4, 3, 4, 4), 1.3)])
+def test_flow_gradient_finite_differences_d3_aniso(S, oracle, obs, dims, xi):
+    """The same finite-difference check in d = 3 and, in d = 4, with the
+    anisotropic weights (xi != 1) of the force and of its adjoint."""
+    U = oracle.random_gauge_trafo(1, dims, oracle.hotstart(1, dims, SEED + 2, 0.6), 654321)
+    eps = 0.02
+    rng = np.random.default_rng(9)
+    with S.Lattice(S.SU2, dims, 2.4, xi=xi) as lat:
+        lat.set_links(U)
+        lat.flow_forward(eps, 3, 2, obs)
+        g = lat.flow_backward()
+        h = 1e-4
+        for _ in range(3):
+            X = rng.normal(size=g.shape)
+            X /= np.linalg.norm(X)
+            fd = (_obs_at(lat, U, X, h, eps, 3, obs) - _obs_at(lat, U, X, -h, eps, 3, obs)) / (2 * h)
+            assert float(np.sum(X * g)) == pytest.approx(fd, rel=1e-6, abs=1e-10)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_kept_stage_states_match_recomputation(S, oracle, prec, monkeypatch):
+    """The backward pass keeps the stage states of a segment from its forward
+    recomputation when memory allows; forcing the per-step recomputation
+    (SU2GPU_FLOW_RECOMPUTE=1) runs the same kernels on the same inputs, so the
+    gradients are identical bit for bit."""
+    dims = (8, 8, 8, 8, 4)  # 3-volume 512: the FP32 stages take the specialised step
+    U = oracle.hotstart(1, dims, SEED, 0.5)
+    grads = []
+    for recompute in ("0", "1"):
+        monkeypatch.setenv("SU2GPU_FLOW_RECOMPUTE", recompute)
+        with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
+            lat.set_links(U)
+            lat.flow_forward(0.01, 23, 10)
+            grads.append(lat.flow_backward())
+    np.testing.assert_array_equal(grads[0], grads[1])
