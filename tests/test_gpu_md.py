@@ -376,16 +376,18 @@ def test_fp32_specialised_step_32(S, oracle):
     assert abs(r32["deltaH"] - r64["deltaH"]) < 0.05
 
 
+# dims = (Lx, Ly, Lz, Lt, nd)
 @pytest.mark.parametrize("dims", [
-    (8, 8, 16, 24, 4),   # t-blocked, Lz = 24: no z-lane exchange
-    (6, 8, 8, 16, 4),    # Lt % 8 != 0: XCD chunks without t-blocking, z-lane exchange
-    (12, 4, 8, 24, 4),   # neither
-    (8, 2, 4, 32, 4),    # Lx = 2, Ly = 4: x and y neighbours wrap within a block
+    (4, 8, 24, 8, 4),    # t-blocked (Lt % 8 == 0), Lz = 24: no z-lane exchange
+    (4, 4, 16, 6, 4),    # Lt % 8 != 0: XCD chunks without t-blocking, z-lane exchange
+    (8, 4, 24, 12, 4),   # neither
+    (2, 4, 32, 8, 4),    # Lx = 2, Ly = 4: x and y neighbours wrap within a block
 ])
 def test_fp32_specialised_step_geometries(S, oracle, dims):
     """The specialised FP32 step on the launch layouts other than 32^4's (every
     3-volume here is a multiple of 256, so the FP32 form takes it) against the
     FP64 form from the same links and host momenta, one OMF2(5) trajectory."""
+    assert (dims[0] * dims[1] * dims[2]) % 256 == 0
     P = oracle.initnormal(1, dims, SEED + 11)
     U = oracle.hotstart(1, dims, SEED + 11, 0.3)
     with S.Lattice(S.SU2, dims, 2.4) as l64:
