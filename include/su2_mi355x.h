@@ -168,6 +168,20 @@ int su2gpu_md_finish(su2gpu_lattice *h, int accept);
 /* energy_density(U, E, Q, cloverdef, ss) (flat-energy_density.hh:57-164) */
 int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, double *Q);
 
+/* ---- Metropolis ------------------------------------------------------------------ */
+/* sweep(U, engines, delta, N_hit, beta, xi, anisotropic) (flat-sweep.hh:64-164):
+ * n_hit Metropolis hits on every link, R = random_element(delta)
+ * (random_element.hh:9-34), dS = beta/Nc (ReTr(U K) - ReTr(U R K)), accept if
+ * dS < 0 or uniform < exp(-dS), U <- restoreSU(U R); anisotropic when xi != 1.
+ * The links are visited per direction mu and site parity (every extent in use
+ * must be even), and the random numbers come from one Philox4x32-10 block per
+ * (link, hit) with key = seed and counter (link, hit, sweep): a sweep depends on
+ * (seed, sweep) only and is not the reference's per-thread mt19937 stream.
+ * *rate = accepted hits / (n_hit * links), *rate_time the same for the
+ * temporal links over n_hit * V: the reference's return values. */
+int su2gpu_sweep(su2gpu_lattice *h, uint64_t seed, uint64_t sweep, double delta, size_t n_hit,
+                 double *rate, double *rate_time);
+
 /* ---- gradient flow (flat-gradient_flow.hh) ------------------------------------- */
 /* runge_kutta(h, SW, eps) (flat-gradient_flow.hh:17-50) applied to the handle's links */
 int su2gpu_runge_kutta(su2gpu_lattice *h, double eps);

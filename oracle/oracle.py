@@ -199,6 +199,25 @@ class Oracle(_Lib):
         for n in ("orc_su2_restore", "orc_su2_exp"):
             getattr(L, n).restype = None
             getattr(L, n).argtypes = [_dp, _dp]
+        L.orc_sweep.restype = None
+        L.orc_sweep.argtypes = [C.c_int, _ip, _dp, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
+                                C.c_double, C.c_long, _dp]
+        L.orc_philox4x32_10.restype = None
+        L.orc_philox4x32_10.argtypes = [_up, _up, _up]
+
+    def sweep(self, group, dims, U, beta, seed, sweep, delta, n_hit, xi=1.0):
+        """Metropolis sweep (flat-sweep.hh:64-164) with the device's order and
+        Philox stream; returns (U_after, (rate, rate_time))."""
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        rates = np.zeros(2)
+        self.lib.orc_sweep(group, dims_arr(dims), U, beta, xi, seed, sweep, delta, n_hit, rates)
+        return U, (rates[0], rates[1])
+
+    def philox(self, ctr, key):
+        out = np.zeros(4, dtype=np.uint32)
+        self.lib.orc_philox4x32_10(np.ascontiguousarray(ctr, dtype=np.uint32),
+                                   np.ascontiguousarray(key, dtype=np.uint32), out)
+        return out
 
     def _rk_args(self):
         return (C.c_int, _ip, _dp, _dp, C.c_double, C.c_double, C.c_double)

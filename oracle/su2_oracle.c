@@ -950,3 +950,162 @@ ORC_API void orc_su2_exp(const double *p, double *out) {
   su2 r = su2_exp(p[0], p[1], p[2]);
   out[0] = r.ar; out[1] = r.ai; out[2] = r.br; out[3] = r.bi;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Metropolis sweep                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+ * as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplications
+ * (0xD2511F53, 0xCD9E8D57) with the Weyl key schedule (0x9E3779B9,
+ * 0xBB67AE85).  Pinned by the paper's known-answer vectors (tests). */
+static void philox4x32_10_c(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+ORC_API void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  philox4x32_10_c(c, key[0], key[1]);
+  for (int k = 0; k < 4; ++k) out[k] = c[k];
+}
+
+static inline double u01_32(uint32_t w) { return ((double)w + 0.5) * (1.0 / 4294967296.0); }
+
+/* the staple sum K of link (s, mu) (get_staples.hh:26-80): SU(2) in
+ * derivative_link's order; U(1) with the anisotropy weight applied to the
+ * exponential of the summed phases */
+static su2 staples_su2(const geom *g, const double *U, size_t s, int mu, double xi, int aniso) {
+  long c[4];
+  coords_of(g, s, c);
+  su2 K = {0, 0, 0, 0};
+  for (int nu = 0; nu < g->nd; ++nu) {
+    if (nu == mu) continue;
+    double f = ((nu == 0) || (mu == 0)) ? 1.0 / xi : xi;
+    su2 u1 = ld_su2(U, lk(g, nb(g, c, mu, 1, -1, 0), nu));
+    if (aniso) u1 = su2_scale(f, u1);
+    su2 t = su2_mul(su2_mul(u1, su2_dag(ld_su2(U, lk(g, nb(g, c, nu, 1, -1, 0), mu)))),
+                    su2_dag(ld_su2(U, lk(g, s, nu))));
+    K = su2_add(K, t);
+  }
+  for (int nu = 0; nu < g->nd; ++nu) {
+    if (nu == mu) continue;
+    double f = ((nu == 0) || (mu == 0)) ? 1.0 / xi : xi;
+    su2 u1 = su2_dag(ld_su2(U, lk(g, nb(g, c, mu, 1, nu, -1), nu)));
+    if (aniso) u1 = su2_scale(f, u1);
+    size_t xm = nb(g, c, nu, -1, -1, 0);
+    su2 t = su2_mul(su2_mul(u1, su2_dag(ld_su2(U, lk(g, xm, mu)))), ld_su2(U, lk(g, xm, nu)));
+    K = su2_add(K, t);
+  }
+  return K;
+}
+static cplx staples_u1(const geom *g, const double *U, size_t s, int mu, double xi, int aniso) {
+  long c[4];
+  coords_of(g, s, c);
+  cplx K = {0, 0};
+  for (int nu = 0; nu < g->nd; ++nu) {
+    if (nu == mu) continue;
+    double th1 = U[lk(g, nb(g, c, mu, 1, -1, 0), nu)];
+    double th2 = U[lk(g, nb(g, c, nu, 1, -1, 0), mu)];
+    double th3 = U[lk(g, s, nu)];
+    double f = (aniso && ((nu == 0) || (mu == 0))) ? 1.0 / xi : (aniso ? xi : 1.0);
+    cplx t = cexpi((th1 + (-th2)) + (-th3));
+    t.re *= f; t.im *= f;
+    K = cadd(K, t);
+  }
+  for (int nu = 0; nu < g->nd; ++nu) {
+    if (nu == mu) continue;
+    size_t xm = nb(g, c, nu, -1, -1, 0);
+    double th1 = U[lk(g, nb(g, c, mu, 1, nu, -1), nu)];
+    double th2 = U[lk(g, xm, mu)];
+    double th3 = U[lk(g, xm, nu)];
+    double f = (aniso && ((nu == 0) || (mu == 0))) ? 1.0 / xi : (aniso ? xi : 1.0);
+    cplx t = cexpi(((-th1) + (-th2)) + th3);
+    t.re *= f; t.im *= f;
+    K = cadd(K, t);
+  }
+  return K;
+}
+
+/* N_hit Metropolis hits on every link (flat-sweep.hh:64-164; one hit
+ * :95-110): R = random_element(delta) (random_element.hh:9-34) from the
+ * uniforms (w + 1/2) 2^-32 of one Philox4x32-10 block with counter
+ * (s nd + mu, hit, sweep_lo, sweep_hi) and key (seed_lo, seed_hi) (alpha, u,
+ * theta, acceptance; U(1): phase, -, -, acceptance), dS = beta/Nc (ReTr(U K) -
+ * ReTr(U R K)), accept if dS < 0 or v < exp(-dS), U <- restoreSU(U R).  Links
+ * are visited per direction mu, then site parity (sum of coordinates mod 2),
+ * as the device sweep does; inside one (mu, parity) set the links are
+ * independent, so the site order does not matter.  rates = {accepted / (N_hit
+ * links), accepted temporal / (N_hit V)} (flat-sweep.hh:160-163). */
+ORC_API void orc_sweep(int grp, const int *dims, double *U, double beta, double xi, uint64_t seed,
+                       uint64_t sweep, double delta, long n_hit, double *rates) {
+  geom g = mkgeom(dims);
+  const int aniso = (xi != 1.0);
+  const double Nc = (grp == 1) ? 2.0 : 1.0;
+  const double beta_nc = beta / Nc;
+  double acc_all = 0, acc_t = 0;
+  for (int mu = 0; mu < g.nd; ++mu) {
+    for (int par = 0; par < 2; ++par) {
+      double acc_mu = 0;
+#pragma omp parallel for schedule(static) reduction(+ : acc_mu)
+      for (size_t s = 0; s < g.V; ++s) {
+        long c[4];
+        coords_of(&g, s, c);
+        if (((c[0] + c[1] + c[2] + c[3]) & 1) != par) continue;
+        const uint32_t lid = (uint32_t)lk(&g, s, mu);
+        if (grp == 1) {
+          const su2 K = staples_su2(&g, U, s, mu, xi, aniso);
+          su2 u = ld_su2(U, lk(&g, s, mu));
+          for (long n = 0; n < n_hit; ++n) {
+            uint32_t w[4] = {lid, (uint32_t)n, (uint32_t)(sweep & 0xffffffffu), (uint32_t)(sweep >> 32)};
+            philox4x32_10_c(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+            const double alpha = delta * (2.0 * ORC_PI) * u01_32(w[0]);
+            const double uu = -1.0 + 2.0 * u01_32(w[1]);
+            const double th = (2.0 * ORC_PI) * u01_32(w[2]);
+            const double r = sqrt(1.0 - uu * uu);
+            double sa, ca, st, ct;
+            sincos(alpha, &sa, &ca);
+            sincos(th, &st, &ct);
+            const su2 R = {ca, sa * uu, (sa * r) * st, (sa * r) * ct};
+            const su2 uR = su2_mul(u, R);
+            const double dS = beta_nc * (2.0 * su2_mul(u, K).ar - 2.0 * su2_mul(uR, K).ar);
+            if (dS < 0.0 || u01_32(w[3]) < exp(-dS)) {
+              u = su2_restore(uR);
+              acc_mu += 1.0;
+            }
+          }
+          st_su2(U, lk(&g, s, mu), u);
+        } else {
+          const cplx K = staples_u1(&g, U, s, mu, xi, aniso);
+          double u = U[lk(&g, s, mu)];
+          for (long n = 0; n < n_hit; ++n) {
+            uint32_t w[4] = {lid, (uint32_t)n, (uint32_t)(sweep & 0xffffffffu), (uint32_t)(sweep >> 32)};
+            philox4x32_10_c(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+            const double th = -ORC_PI * delta + (2.0 * ORC_PI * delta) * u01_32(w[0]);
+            const double un = u + th;
+            const double dS = beta_nc * (cmul(cexpi(u), K).re - cmul(cexpi(un), K).re);
+            if (dS < 0.0 || u01_32(w[3]) < exp(-dS)) {
+              u = un;
+              acc_mu += 1.0;
+            }
+          }
+          U[lk(&g, s, mu)] = u;
+        }
+      }
+      acc_all += acc_mu;
+      if (mu == 0) acc_t += acc_mu;
+    }
+  }
+  const double nh = n_hit > 0 ? (double)n_hit : 1.0;
+  rates[0] = acc_all / nh / (double)(g.V * g.nd);
+  rates[1] = acc_t / nh / (double)g.V;
+}

@@ -83,6 +83,7 @@ _SIGS = {
                                        C.c_int, C.POINTER(MDResult)]),
     "su2gpu_md_finish": (C.c_int, [_h, C.c_int]),
     "su2gpu_energy_density": (C.c_int, [_h, C.c_int, C.c_int, _dp, _dp]),
+    "su2gpu_sweep": (C.c_int, [_h, C.c_uint64, C.c_uint64, C.c_double, C.c_size_t, _dp, _dp]),
     "su2gpu_runge_kutta": (C.c_int, [_h, C.c_double]),
     "su2gpu_gradient_flow": (C.c_int, [_h, C.c_double, C.c_double, C.c_double, _vp, C.c_size_t,
                                        C.POINTER(C.c_size_t)]),
@@ -310,6 +311,13 @@ class Lattice:
         _ck(self.lib.su2gpu_md_finish(self._h, int(accept)))
 
     # -- observables
+    def sweep(self, seed, sweep, delta, n_hit):
+        """n_hit Metropolis hits on every link (flat-sweep.hh:64-164); returns
+        (rate, rate_time) as the reference's sweep does."""
+        r, rt = C.c_double(), C.c_double()
+        _ck(self.lib.su2gpu_sweep(self._h, seed, sweep, delta, n_hit, C.byref(r), C.byref(rt)))
+        return r.value, rt.value
+
     def energy_density(self, clover=True, ss=False):
         E, Q = C.c_double(), C.c_double()
         _ck(self.lib.su2gpu_energy_density(self._h, int(clover), int(ss), C.byref(E), C.byref(Q)))

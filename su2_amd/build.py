@@ -10,7 +10,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsu2mi355x.so")
 
-SOURCES = ["su2_kernels.hip", "su2_flow.hip", "su2_batch.hip", "su2_capi.cpp", "su2_batch.cpp"]
+SOURCES = ["su2_kernels.hip", "su2_flow.hip", "su2_batch.hip", "su2_sweep.hip", "su2_capi.cpp",
+           "su2_batch.cpp"]
 HEADERS = ["su2_device.hpp", "su2_launch.hpp"]
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in any device code

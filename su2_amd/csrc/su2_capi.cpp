@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -1113,6 +1114,48 @@ int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, doub
   if (h->nd == 3) qq = 0.0;
   *E = res;
   *Q = qq;
+  return SU2GPU_OK;
+}
+
+int su2gpu_sweep(su2gpu_lattice *h, uint64_t seed, uint64_t sweep, double delta, size_t n_hit,
+                 double *rate, double *rate_time) {
+  CK(check(h));
+  if (!rate || !rate_time) return fail(SU2GPU_EINVAL, "null pointer");
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
+  for (int mu = 0; mu < h->nd; ++mu)
+    if (h->geo.L[mu] % 2 != 0)
+      return fail(SU2GPU_EINVAL, "sweep: the extent along direction %d (%d) must be even", mu,
+                  h->geo.L[mu]);
+  if (n_hit > (size_t)INT32_MAX) return fail(SU2GPU_EINVAL, "sweep: n_hit too large");
+  // one row of accepted-hit counts per sub-sweep (mu, parity), reduced together
+  constexpr int R0 = 22;
+  static_assert(R0 + 8 <= NROWS, "partial rows");
+  const int nb = su2k::sweep_blocks(h->geo);
+  const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
+  su2k::RowList rl;
+  for (int mu = 0; mu < h->nd; ++mu)
+    for (int par = 0; par < 2; ++par) {
+      {
+        Prof pr(h, "sweep");
+        su2k::launch_sweep(h->group, h->prec, h->U[h->cur], h->geo, mu, par, seed, sweep, delta,
+                           (int)n_hit, h->beta / Nc(h), w_t, w_s, h->aniso ? 1 : 0,
+                           row(h, R0 + 2 * mu + par), h->stream);
+      }
+      CK(last_launch("sweep"));
+      CK(reduce_row(h, R0 + 2 * mu + par, nb, &rl));
+    }
+  CK(reduce_rows(h, rl));
+  CK(fetch_sums(h));
+  long double all = 0.0L, temporal = 0.0L;
+  for (int mu = 0; mu < h->nd; ++mu)
+    for (int par = 0; par < 2; ++par) {
+      const long double a = sum_of(h, R0 + 2 * mu + par);
+      all += a;
+      if (mu == 0) temporal += a;
+    }
+  const long double nh = n_hit > 0 ? (long double)n_hit : 1.0L;
+  *rate = (double)(all / nh / (long double)h->nlinks);
+  *rate_time = (double)(temporal / nh / (long double)h->V);
   return SU2GPU_OK;
 }
 

@@ -85,6 +85,12 @@ void launch_copy16(const void *src, void *dst, long n16, hipStream_t st);
 // gradient-flow adjoint (su2_flow.hip)
 void launch_flow_stage_adj(int prec, const void *Zs, void *lamW, void *lamZ, int V, int nd,
                            double c, int zero_lamZ, hipStream_t st);
+// Metropolis sub-sweep: the links U_mu(x), x of the given parity, n_hit hits
+// each; one accepted-hit count per block to partials[0 .. sweep_blocks(g))
+int sweep_blocks(const Geo &g);
+void launch_sweep(int group, int prec, void *U, const Geo &g, int mu, int parity, uint64_t seed,
+                  uint64_t sweep, double delta, int n_hit, double beta_nc, double w_t, double w_s,
+                  int aniso, double *partials, hipStream_t st);
 // R: scratch of 36 V reals (the b and c role gradients of the 6 planes)
 void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, void *R, const Geo &g,
                      double b, double w_t, double w_s, hipStream_t st);

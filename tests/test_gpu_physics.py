@@ -52,3 +52,29 @@ def test_su2_4d_plaquette_reference_values(S, prec, beta, ref, err):
           f"acceptance {acc / nmeas:.3f}")
     assert acc / nmeas > 0.5
     assert abs(mean - ref) < 4.0 * math.sqrt(e * e + err * err)
+
+
+@pytest.mark.parametrize("beta,ref,err,delta", [(1.0, 0.24320, 0.00005, 0.5),
+                                                (2.0, 0.50133, 0.00012, 0.3)])
+def test_su2_4d_plaquette_reference_values_metropolis(S, beta, ref, err, delta):
+    """The same published plaquettes from a chain of the device Metropolis
+    sweep (flat-sweep.hh:64-164; 10 hits per link), FP64."""
+    dims = (10, 10, 10, 10, 4)
+    ntherm, nmeas, nbin = 200, 2000, 40
+    with S.Lattice(S.SU2, dims, beta) as lat:
+        lat.hotstart(SEED)
+        norm = 2.0 / (4 * 3 * lat.V * 2.0)
+        for i in range(ntherm):
+            lat.sweep(SEED, i, delta, 10)
+        P = np.empty(nmeas)
+        rate = 0.0
+        for i in range(nmeas):
+            rate += lat.sweep(SEED, ntherm + i, delta, 10)[0]
+            P[i] = lat.gauge_energy() * norm
+    bins = P.reshape(nbin, -1).mean(axis=1)
+    mean = bins.mean()
+    e = bins.std(ddof=1) / math.sqrt(nbin)
+    print(f"Metropolis beta={beta}: <P> = {mean:.5f} +- {e:.5f} (ref {ref} +- {err}), "
+          f"acceptance {rate / nmeas:.3f}")
+    assert 0.2 < rate / nmeas < 0.9
+    assert abs(mean - ref) < 4.0 * math.sqrt(e * e + err * err)

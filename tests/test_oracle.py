@@ -306,3 +306,72 @@ def test_flow_decreases_action_and_is_third_order(oracle):
     e1 = np.abs(flow(0.05) - ref).max()
     e2 = np.abs(flow(0.025) - ref).max()
     assert 5.0 < e1 / e2 < 11.0, e1 / e2
+
+
+# ---------------------------------------------------------------- Metropolis --
+PHILOX_KAT = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+              ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+              ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+               [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+
+
+def test_oracle_philox_known_answers(oracle):
+    """The oracle's Philox4x32-10 (the Metropolis random numbers) against the
+    published test vectors of Salmon et al., SC11."""
+    for ctr, key, want in PHILOX_KAT:
+        assert list(oracle.philox(ctr, key)) == want
+
+
+@pytest.mark.parametrize("grp,dims", [(SU2, (4, 4, 4, 4, 4)), (U1, (8, 1, 1, 6, 2)),
+                                      (SU2, (4, 2, 1, 6, 3))])
+def test_oracle_sweep_limits(oracle, grp, dims):
+    """delta = 0 makes R the identity (every hit accepted, links unchanged up to
+    restoreSU); beta = 0 accepts every hit; a sweep is a function of
+    (seed, sweep number) and different sweep numbers give different chains."""
+    U = oracle.hotstart(grp, dims, 11, 1.0)
+    U0, r0 = oracle.sweep(grp, dims, U, 2.0, 5, 0, 0.0, 3)
+    assert r0 == (1.0, 1.0)
+    assert np.abs(U0 - U).max() < 1e-14
+    _, rb = oracle.sweep(grp, dims, U, 0.0, 5, 0, 0.7, 3)
+    assert rb == (1.0, 1.0)
+    Ua, ra = oracle.sweep(grp, dims, U, 2.0, 5, 3, 0.4, 4)
+    Ub, rb2 = oracle.sweep(grp, dims, U, 2.0, 5, 3, 0.4, 4)
+    Uc, _ = oracle.sweep(grp, dims, U, 2.0, 5, 4, 0.4, 4)
+    np.testing.assert_array_equal(Ua, Ub)
+    assert ra == rb2 and 0.0 < ra[0] < 1.0
+    assert np.abs(Uc - Ua).max() > 1e-3
+    if grp == SU2:
+        q = Ua.reshape(-1, 4)
+        assert np.abs((q ** 2).sum(1) - 1.0).max() < 1e-14
+
+
+def test_oracle_sweep_cold_start_frozen_at_large_beta(oracle):
+    """At beta -> infinity every hit away from a cold start raises the action
+    by a finite amount, so the Metropolis test rejects all of them."""
+    dims = (4, 1, 1, 4, 2)
+    U = np.zeros(16 * 2)  # U(1) cold start: ReTr P maximal
+    Uc, r = oracle.sweep(U1, dims, U, 1e9, 3, 0, 0.5, 5)
+    assert r[0] == 0.0
+    np.testing.assert_array_equal(Uc, U)
+
+
+@pytest.mark.parametrize("grp,beta,nu,delta", [(SU2, 2.0, 1, 0.35), (U1, 1.5, 0, 0.6)])
+def test_two_dimensional_plaquette_exact_metropolis(oracle, grp, beta, nu, delta):
+    """d = 2: the Metropolis chain of the oracle's sweep (the device order and
+    Philox stream) reproduces <P> = I_2(b)/I_1(b) (SU(2),
+    doc/reference-values.txt:3-4: 0.43313(2) at b = 2) and <cos> = I_1(b)/I_0(b)
+    (U(1))."""
+    dims = (16, 1, 1, 16, 2)
+    U = oracle.hotstart(grp, dims, 77, 1.0)
+    norm = 1.0 / (16 * 16) / (2.0 if grp == SU2 else 1.0)
+    plaq = []
+    for i in range(1200):
+        U, rate = oracle.sweep(grp, dims, U, beta, 99, i, delta, 8)
+        if i >= 200:
+            plaq.append(oracle.retr_sum_plaq(grp, dims, U) * norm)
+    plaq = np.array(plaq)
+    exact = _bessel_ratio(nu, beta)
+    blocks = plaq.reshape(20, -1).mean(axis=1)
+    err = blocks.std(ddof=1) / math.sqrt(len(blocks))
+    assert abs(plaq.mean() - exact) < 4 * err + 5e-4, (plaq.mean(), exact, err)
+    assert 0.2 < rate[0] < 0.9
