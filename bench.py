@@ -229,6 +229,28 @@ def flow_adjoint(S, L=24, beta=2.4, eps=0.01, tmax=2.0, ckpt=20, ntherm=50, seed
             "grad_norm": float((g ** 2).sum()) ** 0.5}
 
 
+def metropolis(S, L=32, beta=2.4, n_hit=10, delta=0.3, sweeps=4, seed=4243):
+    """Metropolis sweeps (flat-sweep.hh:64-164) at L^4 in both forms: link hits
+    per second and the acceptance rate."""
+    out = {"lattice": [L] * 4, "beta": beta, "n_hit": n_hit, "delta": delta, "sweeps": sweeps}
+    for prec, name in ((S.FP64, "fp64"), (S.FP32, "fp32")):
+        lat = S.Lattice(S.SU2, (L, L, L, L, 4), beta, precision=prec)
+        lat.haarstart(seed)
+        lat.sweep(seed, 0, delta, n_hit)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        rate = 0.0
+        for i in range(sweeps):
+            rate += lat.sweep(seed, 1 + i, delta, n_hit)[0]
+        lat.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"ms_per_sweep": 1000 * dt / sweeps,
+                     "link_hits_per_s": 4 * L ** 4 * n_hit * sweeps / dt,
+                     "acceptance": rate / sweeps}
+        lat.close()
+    return out
+
+
 def cpu_baseline(L, nsteps, beta, seed):
     """The reference's own md_update (oracle/_ref build of its hot-path headers),
     one OMF2 trajectory at the benchmark size on all host cores; falls back to
@@ -299,6 +321,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-flow", action="store_true", help="skip the flow-adjoint leg")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 study")
+    ap.add_argument("--no-metropolis", action="store_true", help="skip the Metropolis-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -316,6 +339,7 @@ def main():
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
     flow = None if args.no_flow else flow_adjoint(S)
+    metro = None if args.no_metropolis else metropolis(S, args.L)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
@@ -355,6 +379,7 @@ def main():
         "fused_compulsory_frac_of_measured_copy_fp32": f32["fused_compulsory_GBs"] / copy_gbs,
         "batch": batch,
         "flow_adjoint": flow,
+        "metropolis": metro,
     }
     print(json.dumps(out), flush=True)
 
