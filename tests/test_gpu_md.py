@@ -505,3 +505,29 @@ omeas:
     lines2 = [ln.split() for ln in out2.stdout.splitlines()
               if ln and ln[0].isdigit() and len(ln.split()) == 7]
     assert [int(x[0]) for x in lines2] == [3, 4]
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+def test_large_lattice_64(S, prec):
+    """64^4 (67 M links; 2.1 GB per FP64 link field): the 32-bit site and link
+    indexing and the specialised step's launch layout at a size 16x the
+    benchmark's, through size-independent properties of one OMF2(10)
+    trajectory with the reversibility test: the action sums of the trajectory
+    equal a separate action evaluation, the reversibility violation stays at
+    the form's rounding floor, the links stay unitary, and a global gauge
+    transformation leaves the action unchanged."""
+    dims = (64, 64, 64, 64, 4)
+    with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
+        lat.hotstart(SEED, 0.3)
+        s0 = lat.gauge_action()
+        r = lat.md_update(SEED + 1, S.OMF2, 10, revtest=True, momenta=S.MOMENTA_DEVICE)
+        assert math.isfinite(r["deltaH"])
+        assert r["gauge_old"] == pytest.approx(s0, rel=1e-12)
+        if r["accept"]:
+            assert r["gauge_new"] == pytest.approx(lat.gauge_action(), rel=1e-12)
+        assert abs(r["deltadeltaH"]) < (1e-6 if prec == 0 else 0.5)
+        s1 = lat.gauge_action()
+        lat.random_gauge_trafo(654321)
+        assert lat.gauge_action() == pytest.approx(s1, rel=1e-9 if prec == 0 else 1e-5)
+        q = lat.get_links().reshape(-1, 4)
+        assert np.abs((q ** 2).sum(1) - 1.0).max() < (1e-14 if prec == 0 else 5e-7)
