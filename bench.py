@@ -344,7 +344,7 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
     n = args.nsteps
-    kname = "k_md_su2_4d<float,true,false,1,true>"
+    kname = "k_md_su2_4d<float,true,false,1,true,false>"
     out = {
         "metric": "SU(2) 32^4 HMC link-updates/s, %HBM roofline; FP32 vs FP64 form at |dDH|<=1e-6",
         "value": f32["link_updates_per_s"],

@@ -460,20 +460,32 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
   } else {
     const int lvl = fast_level();
     if (lvl == 0 || (sizeof(T) == 8 && lvl < 2)) return false;
-    if (L.c.flags & (F_PZERO | F_ROUND | F_ANISO)) return false;
+    if (L.c.flags & (F_PZERO | F_ROUND)) return false;
     // the thread-per-site shape keeps its own partial-sum count (fused_blocks)
     if (fused_mode(L.geo) == 0) return false;
     FastGeo f;
     if (!make_fast_geo(L.geo.L, L.geo.V, tblock_pref() != 0, f)) return false;
     const bool acc = (L.c.flags & F_ACCUM) != 0;
     const bool site = fast_shape() == 1;
+    // the thread-per-link shape weights each plane (w_t, w_s); the per-site one
+    // sums unweighted
+    if (site && (L.c.flags & F_ANISO)) return false;
     // +-z neighbours by lane exchange where every wave holds whole z-rows
     const bool zs = !site && zs_pref() && (64 % f.Lz == 0);
+    const bool an = (L.c.flags & F_ANISO) != 0;
     dim3 grid((site ? 1 : 4) * (L.geo.V / NTF)), block(NTF);
 #define SU2_FAST(W, A)                                                                          \
   if (site)                                                                                     \
     hipLaunchKernelGGL((k_md_su2_4d_site<T, W, A>), grid, block, 0, st, (const quat<T> *)L.Uin, \
                        (quat<T> *)L.Uout, (T *)L.P, f, L.c, L.partials, L.pstride);             \
+  else if (an && zs)                                                                            \
+    hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, true, true>), grid, block, 0, st,              \
+                       (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
+                       L.partials, L.pstride);                                                  \
+  else if (an)                                                                                  \
+    hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, false, true>), grid, block, 0, st,             \
+                       (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
+                       L.partials, L.pstride);                                                  \
   else if (zs)                                                                                  \
     hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, true>), grid, block, 0, st,                    \
                        (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \

@@ -185,6 +185,16 @@ __device__ __forceinline__ void acc_d(double (&K)[4], const quat<T> &q) {
   K[2] += double(q.br);
   K[3] += double(q.bi);
 }
+// K += w q in one rounding per component: the anisotropy weight of the plane
+// (get_staples.hh:58-79) costs nothing over the plain sum (an FMA for an add;
+// w = 1 gives the plain sum bit for bit)
+template <typename T>
+__device__ __forceinline__ void acc_dw(double (&K)[4], double w, const quat<T> &q) {
+  K[0] = fma(w, double(q.ar), K[0]);
+  K[1] = fma(w, double(q.ai), K[1]);
+  K[2] = fma(w, double(q.br), K[2]);
+  K[3] = fma(w, double(q.bi), K[3]);
+}
 
 
 // Momentum and link update of one link from its staple sum K (FP64):
@@ -248,18 +258,27 @@ template <typename T> __device__ __forceinline__ double sq3(T a, T b, T c) {
   return double(a) * double(a) + double(b) * double(b) + double(c) * double(c);
 }
 
-// upper + lower staple of one plane into K (FP64): the reference's
-// U_nu(s+mu) U_mu(s+nu)^+ U_nu(s)^+ and U_nu(s+mu-nu)^+ U_mu(s-nu)^+ U_nu(s-nu)
-template <typename T>
-__device__ __forceinline__ void staple_pair(double (&K)[4], const quat<T> &A, const quat<T> &B,
-                                            const quat<T> &C, const quat<T> &D, const quat<T> &E,
-                                            const quat<T> &F) {
-  if (sizeof(T) == 4) {
-    acc_d(K, qmul_fma(qmul_fma(A, qdag(B)), qdag(C)));
-    acc_d(K, qmul_fma(qmul_fma(qdag(D), qdag(E)), F));
+// upper + lower staple of one plane, weighted by w, into K (FP64): the
+// reference's U_nu(s+mu) U_mu(s+nu)^+ U_nu(s)^+ and U_nu(s+mu-nu)^+ U_mu(s-nu)^+ U_nu(s-nu)
+// (AN = false: w is 1 and unused, the plain sum)
+template <typename T, bool AN>
+__device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat<T> &A,
+                                            const quat<T> &B, const quat<T> &C, const quat<T> &D,
+                                            const quat<T> &E, const quat<T> &F) {
+  quat<T> up, lo;
+  if constexpr (sizeof(T) == 4) {
+    up = qmul_fma(qmul_fma(A, qdag(B)), qdag(C));
+    lo = qmul_fma(qmul_fma(qdag(D), qdag(E)), F);
   } else {
-    acc_d(K, GSU2<T>::upper(A, B, C));
-    acc_d(K, GSU2<T>::lower(D, E, F));
+    up = GSU2<T>::upper(A, B, C);
+    lo = GSU2<T>::lower(D, E, F);
+  }
+  if (AN) {
+    acc_dw(K, w, up);
+    acc_dw(K, w, lo);
+  } else {
+    acc_d(K, up);
+    acc_d(K, lo);
   }
 }
 
@@ -267,7 +286,7 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], const quat<T> &A, co
 // three planes nu != mu are known at compile time and all 18 staple links are
 // loaded before any arithmetic (one memory round trip per thread).  MU < 0:
 // mu at run time, one round trip per plane.
-template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false>
+template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false>
 __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
@@ -335,13 +354,18 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
       }
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) staple_pair(K, A[k], B[k], C[k], D[k], E[k], F[k]);
+    for (int k = 0; k < 3; ++k) {
+      const int nu = k + (k >= MU ? 1 : 0);
+      staple_pair<T, AN>(K, (MU == 0 || nu == 0) ? c.w_t : c.w_s, A[k], B[k], C[k], D[k], E[k],
+                         F[k]);
+    }
   } else {
 #pragma unroll
     for (int nu = 0; nu < 4; ++nu) {
       if (nu == mu) continue;
       const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
-      staple_pair(K, ldo(Un, (uint32_t)(s + up[mu])), ldo(Um, (uint32_t)(s + up[nu])),
+      staple_pair<T, AN>(K, (mu == 0 || nu == 0) ? c.w_t : c.w_s, ldo(Un, (uint32_t)(s + up[mu])),
+                  ldo(Um, (uint32_t)(s + up[nu])),
                   ldo(Un, (uint32_t)s), ldo(Un, (uint32_t)(s + up[mu] + dn[nu])),
                   ldo(Um, (uint32_t)(s + dn[nu])), ldo(Un, (uint32_t)(s + dn[nu])));
     }
@@ -364,7 +388,8 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
 // SPEC: 1 = the block's mu selects one of four bodies with mu fixed at compile
 // time (mu is uniform per block); 0 = one body with mu at run time
 // ZS (with SPEC; needs 64 % Lz == 0): +-z neighbours by lane exchange
-template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false>
+// AN: anisotropic weights w_t / w_s on the planes (FMA accumulation)
+template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false, bool AN = false>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ Uin,
                                                  quat<T> *__restrict__ Uout,
                                                  T *__restrict__ P, FastGeo g, FusedCoef c,
@@ -373,13 +398,13 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ U
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
   if (SPEC) {
     switch (fs.mu) {
-      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
     }
   } else {
-    md_su2_body<T, -1, WRITE_U, ACCUM>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
+    md_su2_body<T, -1, WRITE_U, ACCUM, false, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
   }
 }
 

@@ -412,6 +412,26 @@ def test_fp32_specialised_step_geometries(S, oracle, dims):
     assert np.abs((q ** 2).sum(1) - 1.0).max() < 5e-7
 
 
+@pytest.mark.parametrize("xi", [1.3, 0.8])
+def test_fp32_specialised_step_anisotropic(S, oracle, xi):
+    """The specialised FP32 step with the anisotropic plane weights (w_t = 1/xi
+    on planes touching t, w_s = xi on spatial ones, get_staples.hh:58-79)
+    against the oracle's FP64 trajectory from the same links and momenta."""
+    dims = (8, 8, 8, 8, 4)
+    U = oracle.hotstart(1, dims, SEED + 5, 0.3)
+    P = oracle.initnormal(1, dims, SEED + 6)
+    Uw, Pw = oracle.integrate(1, dims, U, P, 2.4, 6, 5, xi=xi)
+    h0 = oracle.action(1, dims, U, 2.4, xi=xi, compensated=True) + oracle.kinetic(1, dims, P, compensated=True)
+    h1 = oracle.action(1, dims, Uw, 2.4, xi=xi, compensated=True) + oracle.kinetic(1, dims, Pw, compensated=True)
+    with S.Lattice(S.SU2, dims, 2.4, xi=xi, precision=S.FP32) as lat:
+        lat.set_links(U)
+        r = lat.md_trajectory(P, 6, 5)
+        lat.md_finish(True)
+        Ug = lat.get_links()
+    assert np.abs(Ug - Uw).max() < 2e-5
+    assert abs(r["deltaH"] - (h1 - h0)) < 2e-3
+
+
 def test_fp64_trajectory_parity_32(S, oracle):
     """The plain FP64 form at the benchmark size: one OMF2(20) trajectory on
     32^4 at beta = 2.4 from the seeded hot start with the reference's host
