@@ -173,8 +173,9 @@ def batch_ensemble(S, steps, warmup):
     d=4), FP32, one launch per MD tick, hipGraph replay, autotuned launch shape."""
     members = S.ensemble_members()
     b = S.Batch(members, precision=S.FP32)
-    cache = os.path.join(ROOT, "gpurun_out", "batch_tune.json")
-    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    import tempfile
+    # a fresh cache per run, so that the tuning time is measured every time
+    cache = os.path.join(tempfile.mkdtemp(prefix="su2_tune_"), "batch_tune.json")
     t0 = time.perf_counter()
     blk, grd = b.autotune(cache)
     t_tune = time.perf_counter() - t0
