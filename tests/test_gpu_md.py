@@ -265,6 +265,26 @@ def test_fp32_form_tracks_fp64(S, oracle):
     assert np.abs(Ug - Uw).max() < 1e-4
 
 
+@pytest.mark.parametrize("grp,dims,tol", [(0, (16, 1, 1, 12, 2), 1e-5), (1, (6, 4, 1, 8, 3), 1e-4),
+                                          (0, (4, 6, 4, 6, 4), 1e-5)])
+def test_fp32_form_u1_and_lower_dimensions(S, oracle, grp, dims, tol):
+    """The FP32 form on the generic kernels (U(1); SU(2) in d = 3; U(1) in
+    d = 4) against the oracle's FP64 trajectory from the same start and host
+    momenta."""
+    U = oracle.hotstart(grp, dims, SEED + 9, 0.4)
+    P = oracle.initnormal(grp, dims, SEED + 10)
+    Uw, Pw = oracle.integrate(grp, dims, U, P, 2.0, 6, 8)
+    h0 = oracle.action(grp, dims, U, 2.0, compensated=True) + oracle.kinetic(grp, dims, P, compensated=True)
+    h1 = oracle.action(grp, dims, Uw, 2.0, compensated=True) + oracle.kinetic(grp, dims, Pw, compensated=True)
+    with S.Lattice(grp, dims, 2.0, precision=S.FP32) as lat:
+        lat.set_links(U)
+        r = lat.md_trajectory(P, 6, 8)
+        lat.md_finish(True)
+        Ug = lat.get_links()
+    assert np.abs(Ug - Uw).max() < tol
+    assert abs(r["deltaH"] - (h1 - h0)) < 1e-3
+
+
 @pytest.mark.parametrize("dims", [(8, 8, 8, 8, 4), (16, 8, 8, 8, 4), (4, 4, 4, 4, 4)])
 @pytest.mark.parametrize("kind", [6, 0])
 def test_fp32_trajectory_action_sums(S, oracle, dims, kind):
