@@ -289,6 +289,25 @@ void gradient_flow(const gaugeconfig<G> &U, std::string const &path, const doubl
     for (int c = 0; c < 9; ++c) ofs << r[c] << (c < 8 ? " " : "\n");
   }
 }
+// sweep(U, engines, delta, N_hit, beta, xi, anisotropic) (flat-sweep.hh:64-164):
+// N_hit Metropolis hits on every link.  The caller's engines (a copy, as the
+// reference takes them by value; metropolis.hpp:96-104 seeds them with
+// seed + i + thread per sweep i) supply the key of the device's Philox stream:
+// two draws of engines[0] make a 64-bit key, so the chain is reproducible from
+// the caller's seeds as the reference's is, though with other draws.  beta
+// becomes the lattice's beta; xi and the anisotropy are the lattice's own.
+// Returns {rate, rate_time} as the reference does.
+template <class URNG, class G>
+std::vector<double> sweep(gaugeconfig<G> &U, std::vector<URNG> engines, const double &delta,
+                          const size_t &N_hit, const double &beta, const double & /*xi*/ = 1.0,
+                          const bool & /*anisotropic*/ = false) {
+  if (engines.empty()) throw gpu_error("sweep: no engine");
+  const uint64_t hi = (uint64_t)(uint32_t)engines[0](), lo = (uint64_t)(uint32_t)engines[0]();
+  if (U.getBeta() != beta) U.setBeta(beta);
+  double rate = 0.0, rate_time = 0.0;
+  check(su2gpu_sweep(U.handle(), (hi << 32) | lo, 0, delta, N_hit, &rate, &rate_time), "sweep");
+  return {rate, rate_time};
+}
 }  // namespace flat_spacetime
 
 // integrators (integrator.hh:15-312)

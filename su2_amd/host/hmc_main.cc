@@ -17,6 +17,10 @@
 //     precision: fp64 | fp32
 //
 //   usage: su2_hmc_gpu -f input.yaml [-g su2|u1]
+//
+// An input with a `metropolis:` section runs the Metropolis program instead
+// (metropolis.hpp; keys n_meas, N_hit, delta, seed, heat, N_save, conf_dir,
+// conf_basename, lenghty_conf_name, beta_str_width).
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -231,6 +235,89 @@ template <class G> int run(const Map &in) {
   return 0;
 }
 
+// Metropolis program: metropolis<G>::run (metropolis.hpp:158-189) with
+// do_sweep (:96-131) for one "thread": per sweep i, engine mt19937(seed + i),
+// sweep(U, engines, delta, N_hit, beta, xi, anisotropic), then the line
+// "i P E Q P_ss E_ss Q_ss" (energy_density without the clover definition),
+// a configuration every N_save sweeps and the acceptance rates at the end.
+template <class G> int run_metropolis(const Map &in) {
+  namespace S = su2_mi355x;
+  size_t Lx, Ly, Lz;
+  if (in.count("geometry.L")) {
+    Lx = Ly = Lz = get<size_t>(in, "geometry.L", 0);
+  } else {
+    Lx = get<size_t>(in, "geometry.X", 0);
+    Ly = get<size_t>(in, "geometry.Y", 0);
+    Lz = get<size_t>(in, "geometry.Z", 0);
+  }
+  const size_t Lt = get<size_t>(in, "geometry.T", 0);
+  const size_t nd = get<size_t>(in, "geometry.ndims", 4);
+  if (nd < 4) Lz = 1;
+  if (nd < 3) Ly = 1;
+  const double beta = get<double>(in, "monomials.gauge.beta", 0.0);
+  const double xi = get<double>(in, "monomials.gauge.anisotropic.xi", 1.0);
+  const bool aniso = xi != 1.0;
+  const size_t n_meas = get<size_t>(in, "metropolis.n_meas", 10);
+  const size_t N_save = get<size_t>(in, "metropolis.N_save", 100);
+  const size_t seed = get<size_t>(in, "metropolis.seed", 13526463);
+  const size_t N_hit = get<size_t>(in, "metropolis.N_hit", 10);
+  const double delta = get<double>(in, "metropolis.delta", 1.0);
+  const bool heat = getb(in, "metropolis.heat", true);
+  if (getb(in, "metropolis.restart", false)) {
+    std::cerr << "Error: metropolis.restart is not supported by this program\n";
+    return 1;
+  }
+  std::string conf_dir = gets(in, "metropolis.conf_dir", ".");
+  const std::string basename = gets(in, "metropolis.conf_basename", "conf");
+  const bool lengthy = getb(in, "metropolis.lenghty_conf_name", true);
+  const int bw = get<int>(in, "metropolis.beta_str_width", 6);
+  const int prec = gets(in, "gpu.precision", "fp64") == "fp32" ? SU2GPU_FP32 : SU2GPU_FP64;
+
+  std::cout << "## Metropolis Algorithm on MI355X (gfx950)\n";
+  mkdir(conf_dir.c_str(), 0755);
+  std::stringstream ss;
+  ss << basename;
+  if (lengthy) {
+    ss << "." << Lx << "." << Ly << "." << Lz << "." << Lt << ".b" << std::fixed
+       << std::setprecision(bw) << beta;
+    if (xi != 1.0) ss << ".x" << std::fixed << std::setprecision(bw) << xi;
+  }
+  const std::string conf_path_basename = conf_dir + "/" + ss.str();
+
+  S::gaugeconfig<G> U(Lx, Ly, Lz, Lt, nd, beta, xi, prec);
+  S::hotstart(U, (int)seed, heat);
+  S::random_gauge_trafo(U, 654321);
+  std::ofstream os(conf_dir + "/output.u1-metropolis.data", std::ios::out);
+  os << "## i P E Q P_ss E_ss Q_ss\n";
+  const double normalisation = 2.0 / nd / (nd - 1) / U.getVolume() / double(U.getNc());
+  const double facnorm = (nd > 2) ? double(nd) / double(nd - 2) : 0.0;
+  double rate[2] = {0.0, 0.0};
+  for (size_t i = 0; i < n_meas; ++i) {
+    std::vector<std::mt19937> engines(1);
+    engines[0].seed(seed + i);
+    const std::vector<double> r = S::flat_spacetime::sweep(U, engines, delta, N_hit, beta, xi, aniso);
+    rate[0] += r[0];
+    rate[1] += r[1];
+    std::ostringstream line;
+    line << i;
+    for (bool sp : {false, true}) {
+      double E = 0., Q = 0.;
+      S::flat_spacetime::energy_density(U, E, Q, false, sp);
+      const double energy = S::flat_spacetime::gauge_energy(U, sp);
+      line << " " << std::scientific << std::setprecision(15)
+           << energy * normalisation * (sp ? facnorm : 1.0) << " " << E << " " << Q;
+    }
+    line << "\n";
+    std::cout << line.str() << std::flush;
+    os << line.str() << std::flush;
+    if (i > 0 && (i % N_save) == 0) U.save(conf_path_basename + "." + std::to_string(i));
+  }
+  std::cout << "## Acceptance rate " << rate[0] / double(n_meas) << " temporal acceptance rate "
+            << rate[1] / double(n_meas) << std::endl;
+  U.save(conf_path_basename + ".final");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -246,7 +333,14 @@ int main(int argc, char **argv) {
   }
   std::cout << "## Parsing input file: " << input << "\n";
   const Map in = parse_yaml_subset(input);
+  bool metropolis = false;
+  for (const auto &kv : in)
+    if (kv.first.rfind("metropolis.", 0) == 0) metropolis = true;
   try {
+    if (metropolis) {
+      if (group == "u1") return run_metropolis<su2_mi355x::_u1>(in);
+      return run_metropolis<su2_mi355x::_su2>(in);
+    }
     if (group == "u1") return run<su2_mi355x::_u1>(in);
     return run<su2_mi355x::_su2>(in);
   } catch (const std::exception &e) {

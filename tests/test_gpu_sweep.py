@@ -61,3 +61,50 @@ def test_sweep_needs_even_extents(S):
         lat.coldstart()
         with pytest.raises(S.SU2GPUError, match="even"):
             lat.sweep(1, 0, 0.4, 1)
+
+
+def test_host_program_metropolis(S, oracle, tmp_path):
+    """The C++ host program in its Metropolis mode (metropolis.hpp's loop over
+    the adapter's flat_spacetime::sweep, which keys the device stream with two
+    draws of the caller's mt19937(seed + i)) against the same chain driven
+    through the C ABI from Python; rows "i P E Q P_ss E_ss Q_ss"."""
+    import os
+    import subprocess
+    from su2_amd import host
+    exe = host.EXE
+    assert os.path.exists(exe), "host program not built"
+    seed, n_meas, n_hit, delta, beta = 4321, 5, 4, 0.4, 2.2
+    inp = tmp_path / "in.yaml"
+    inp.write_text(f"""geometry:
+  L : 4
+  T : 4
+  ndims : 4
+metropolis:
+  seed: {seed}
+  n_meas: {n_meas}
+  N_hit: {n_hit}
+  delta: {delta}
+  heat: true
+  lenghty_conf_name: false
+  conf_dir: "{tmp_path / 'confs'}/"
+monomials:
+  gauge:
+    beta: {beta}
+""")
+    out = subprocess.run([exe, "-f", str(inp)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [ln.split() for ln in out.stdout.splitlines()
+            if ln and ln[0].isdigit() and len(ln.split()) == 7]
+    assert len(rows) == n_meas
+    with S.Lattice(S.SU2, (4, 4, 4, 4, 4), beta) as lat:
+        lat.hotstart(seed, 1.0)
+        lat.random_gauge_trafo(654321)
+        norm = 2.0 / (4 * 3 * lat.V * 2.0)
+        for i, row in enumerate(rows):
+            w = oracle.mt_words(seed + i, 2)
+            lat.sweep((int(w[0]) << 32) | int(w[1]), 0, delta, n_hit)
+            assert int(row[0]) == i
+            assert float(row[1]) == pytest.approx(lat.gauge_energy() * norm, rel=1e-13)
+            E, Q = lat.energy_density(clover=False)
+            assert float(row[2]) == pytest.approx(E, rel=1e-13)
+    assert "## Acceptance rate" in out.stdout
