@@ -402,6 +402,7 @@ def test_fp32_specialised_step_32(S, oracle):
     (4, 4, 16, 6, 4),    # Lt % 8 != 0: XCD chunks without t-blocking, z-lane exchange
     (8, 4, 24, 12, 4),   # neither
     (2, 4, 32, 8, 4),    # Lx = 2, Ly = 4: x and y neighbours wrap within a block
+    (16, 16, 1, 8, 4),   # Lz = 1: the z neighbours are the link itself (lane exchange with itself)
 ])
 def test_fp32_specialised_step_geometries(S, oracle, dims):
     """The specialised FP32 step on the launch layouts other than 32^4's (every
