@@ -24,9 +24,11 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../su2_mi355x.h"
+#include "normals.hh"
 
 namespace su2_mi355x {
 
@@ -180,8 +182,14 @@ private:
 
 template <class URNG, typename Float, class G>
 void initnormal(URNG &engine, adjointfield<Float, G> &A) {
-  std::normal_distribution<double> normal(0., 1.);
-  for (auto &x : A.data) x = Float(normal(engine));
+  if constexpr (std::is_same<URNG, std::mt19937>::value && std::is_same<Float, double>::value) {
+    // the same draws, two passes (normals.hh)
+    std::vector<double> pairs;
+    detail::polar_normals(engine, A.data.data(), A.data.size(), pairs);
+  } else {
+    std::normal_distribution<double> normal(0., 1.);
+    for (auto &x : A.data) x = Float(normal(engine));
+  }
 }
 
 template <typename Float, class G> struct hamiltonian_field {

@@ -14,12 +14,15 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/su2_mi355x.h"
+#include "../../include/su2_mi355x/normals.hh"
 #include "su2_launch.hpp"
 
 namespace {
@@ -70,6 +73,7 @@ struct su2gpu_lattice {
   double *staging = nullptr;
   size_t staging_cap = 0;
   double *hstage = nullptr;
+  std::vector<double> hpairs;  // accepted polar-method pairs of host_normals
   size_t hstage_cap = 0;
   hipStream_t stream = nullptr;
   int device = 0;
@@ -568,10 +572,10 @@ int get_mom_host(L *h, const void *srcP, double *dst) {
 }
 
 // initnormal (adjointfield.hh:240-258) on the host into hstage
+// initnormal (adjointfield.hh:240-258) of the handle's momenta on the host:
+// the reference's normal stream bit for bit (two passes, normals.hh)
 void host_normals(L *h, std::mt19937 &engine) {
-  std::normal_distribution<double> normal(0., 1.);
-  const size_t n = (size_t)h->nlinks * h->nm;
-  for (size_t i = 0; i < n; ++i) h->hstage[i] = normal(engine);
+  su2_mi355x::detail::polar_normals(engine, h->hstage, (size_t)h->nlinks * h->nm, h->hpairs);
 }
 
 int check(const L *h) {
