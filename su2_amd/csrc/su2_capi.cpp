@@ -467,14 +467,16 @@ int run_integrator(L *h, int kind, size_t nsteps, double tau, size_t exponent,
 // host-side random start configurations with the reference's exact streams
 constexpr double ref_pi() { return 3.14159265358979311599796346854; }  // atan(1)*4
 
+// uniform_real_distribution<double>(a, b)(e) of libstdc++: u * (b - a) + a with
+// u = generate_canonical<double, 53>(e) (canonical53, normals.hh)
+inline double uniform_ab(std::mt19937 &e, double a, double b) {
+  return (su2_mi355x::detail::canonical53(e) * (b - a)) + a;
+}
 void random_su2(std::mt19937 &e, double delta, double *q) {
   // random_element (random_element.hh:9-25)
-  std::uniform_real_distribution<double> dist1(-1., 1.);
-  std::uniform_real_distribution<double> dist2(0., 2 * ref_pi());
-  std::uniform_real_distribution<double> dist3(0., delta * 2 * ref_pi());
-  const double alpha = dist3(e);
-  const double u = dist1(e);
-  const double theta = dist2(e);
+  const double alpha = uniform_ab(e, 0., delta * 2 * ref_pi());
+  const double u = uniform_ab(e, -1., 1.);
+  const double theta = uniform_ab(e, 0., 2 * ref_pi());
   const double r = std::sqrt(1 - u * u);
   // the reference build evaluates each sin/cos pair with one glibc sincos call
   // (GCC merges them); sincos and separate sin/cos can differ in the last bit
@@ -488,8 +490,7 @@ void random_su2(std::mt19937 &e, double delta, double *q) {
   q[3] = sr * ct;
 }
 double random_u1(std::mt19937 &e, double delta) {
-  std::uniform_real_distribution<double> dist(-ref_pi() * delta, ref_pi() * delta);
-  return dist(e);
+  return uniform_ab(e, -ref_pi() * delta, ref_pi() * delta);
 }
 
 // (a1 + i b1)(a2 + i b2) in std::complex order
@@ -806,20 +807,19 @@ int su2gpu_haarstart(su2gpu_lattice *h, uint64_t seed) {
   const size_t n = (size_t)h->nlinks * h->ncomp;
   CK(ensure_staging(h, n));
   std::mt19937 engine(seed);
-  std::normal_distribution<double> normal(0., 1.);
-  std::uniform_real_distribution<double> uni(-ref_pi(), ref_pi());
-  for (long i = 0; i < h->nlinks; ++i) {
-    if (h->group == SU2GPU_SU2) {
-      double q[4], r2 = 0.0;
-      for (int k = 0; k < 4; ++k) {
-        q[k] = normal(engine);
-        r2 += q[k] * q[k];
-      }
+  if (h->group == SU2GPU_SU2) {
+    // four normals per link in stream order (the one-by-one normal_distribution
+    // draws, normals.hh), normalised
+    su2_mi355x::detail::polar_normals(engine, h->hstage, n, h->hpairs);
+    for (long i = 0; i < h->nlinks; ++i) {
+      double *q = h->hstage + 4 * i;
+      double r2 = 0.0;
+      for (int k = 0; k < 4; ++k) r2 += q[k] * q[k];
       const double r = std::sqrt(r2);
-      for (int k = 0; k < 4; ++k) h->hstage[4 * i + k] = q[k] / r;
-    } else {
-      h->hstage[i] = uni(engine);
+      for (int k = 0; k < 4; ++k) q[k] = q[k] / r;
     }
+  } else {
+    for (long i = 0; i < h->nlinks; ++i) h->hstage[i] = uniform_ab(engine, -ref_pi(), ref_pi());
   }
   return set_links_host(h, h->hstage, h->ncomp, 0);
 }

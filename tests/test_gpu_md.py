@@ -47,6 +47,16 @@ def test_start_configuration_bit_exact(S, oracle, golden):
         np.testing.assert_array_equal(lat.get_links(), golden["u1_88_start"])
 
 
+@pytest.mark.parametrize("grp,dims", [(1, (4, 4, 4, 4, 4)), (1, (16, 16, 16, 8, 4)),
+                                      (0, (8, 1, 1, 6, 2))])
+def test_haarstart_bit_exact(S, oracle, grp, dims):
+    """The Haar-uniform start (four normal_distribution draws per SU(2) link,
+    normalised; a uniform phase per U(1) link) against the oracle's."""
+    with S.Lattice(grp, dims, 2.3) as lat:
+        lat.haarstart(SEED)
+        np.testing.assert_array_equal(lat.get_links(), oracle.haarstart(grp, dims, SEED))
+
+
 @pytest.mark.parametrize("dims", [(4, 4, 4, 4, 4), (16, 16, 16, 16, 4)])
 def test_initnormal_bit_exact(S, oracle, dims):
     """The reference's momentum stream (mt19937 + libstdc++'s polar
