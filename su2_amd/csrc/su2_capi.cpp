@@ -1420,6 +1420,11 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
       }
     }
   }
+  // The gather of a stage's Hessian-vector product (the b/c role terms in R)
+  // runs fused with the pointwise adjoint of the stage before it; the stage
+  // recomputation in between touches neither R nor lamW / lamZ.
+  bool pending = false;
+  double pending_b = 0.0;
   const size_t nseg = f.ckpts.size();
   for (size_t sg = nseg; sg-- > 0;) {
     const size_t k0 = sg * f.ckpt, k1 = std::min(f.nsteps, k0 + f.ckpt);
@@ -1429,13 +1434,16 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
       // stage by stage, keeping W1, W2 and the three Z'
       for (size_t k = k0; k < k1; ++k) {
         const size_t j = k - k0;
-        CK(rk_stage(h, f.seg[j], f.stU[2 * j], f.Z, b[0], c[0], true));
-        HC(hipMemcpyAsync(f.stZ[3 * j], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
-        CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.Z, b[1], c[1], false));
-        HC(hipMemcpyAsync(f.stZ[3 * j + 1], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
-        CK(rk_stage(h, f.stU[2 * j + 1], k + 1 < k1 ? f.seg[j + 1] : nullptr, f.Z, b[2], c[2],
-                    false));
-        HC(hipMemcpyAsync(f.stZ[3 * j + 2], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
+        // stage 0 ignores its Z input and writes Z0' in place; stages 1 and 2
+        // update a copy of the previous Z'
+        CK(rk_stage(h, f.seg[j], f.stU[2 * j], f.stZ[3 * j], b[0], c[0], true));
+        HC(hipMemcpyAsync(f.stZ[3 * j + 1], f.stZ[3 * j], h->P_bytes, hipMemcpyDeviceToDevice,
+                          h->stream));
+        CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.stZ[3 * j + 1], b[1], c[1], false));
+        HC(hipMemcpyAsync(f.stZ[3 * j + 2], f.stZ[3 * j + 1], h->P_bytes,
+                          hipMemcpyDeviceToDevice, h->stream));
+        CK(rk_stage(h, f.stU[2 * j + 1], k + 1 < k1 ? f.seg[j + 1] : nullptr, f.stZ[3 * j + 2],
+                    b[2], c[2], false));
       }
     } else {
       for (size_t k = k0; k + 1 < k1; ++k)
@@ -1464,18 +1472,28 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
       for (int st = 2; st >= 0; --st) {
         {
           Prof pr(h, "flow_stage_adj");
-          su2k::launch_flow_stage_adj(h->prec, Zst[st], f.lamW, f.lamZ, h->V, h->nd, c[st],
-                                      st == 2 ? 1 : 0, h->stream);
+          if (pending)
+            su2k::launch_flow_gather_stage_adj(h->prec, f.R, f.lamW, f.lamZ, Zst[st], h->geo,
+                                               pending_b, c[st], st == 2 ? 1 : 0, h->stream);
+          else
+            su2k::launch_flow_stage_adj(h->prec, Zst[st], f.lamW, f.lamZ, h->V, h->nd, c[st],
+                                        st == 2 ? 1 : 0, h->stream);
         }
         CK(last_launch("flow_stage_adj"));
         {
           Prof pr(h, "flow_vjp");
           su2k::launch_flow_vjp(h->prec, Wst[st], f.lamZ, f.lamW, f.R, h->geo, b[st], w_t, w_s,
-                                h->stream);
+                                false, h->stream);
         }
         CK(last_launch("flow_vjp"));
+        pending = true;
+        pending_b = b[st];
       }
     }
+  }
+  if (pending) {
+    su2k::launch_flow_gather(h->prec, f.R, f.lamW, h->geo, pending_b, h->stream);
+    CK(last_launch("flow_gather"));
   }
   return get_mom_host(h, f.lamW, grad);
 }

@@ -15,7 +15,8 @@
 //
 // (dD/dW)^T lambda collects, per link, one term from each of its 2(d-1)
 // plaquettes; every plaquette is evaluated once and gives the terms of its four
-// links (k_flow_vjp_plaq, k_flow_vjp_gather).  For the
+// links (k_flow_vjp_plaq, then k_flow_vjp_gather or, fused with the pointwise
+// adjoint of the next stage to run, k_flow_gather_stage_adj).  For the
 // plaquette P = a b c^+ d^+ (a = U_mu(x), b = U_nu(x+mu), c = U_mu(x+nu),
 // d = U_nu(x)) sum_m lambda_m . D_m restricted to P equals -2 Re(P Omega) with
 // Omega = L_a + a L_b a^+ - d L_c d^+ - L_d (L = pure quaternion of lambda), and
@@ -65,18 +66,12 @@ template <typename T> __device__ __forceinline__ v3<T> ld3(const T *P, int plane
 }
 
 // ---------------------------------------------------------------------------
-// pointwise part of one stage adjoint (see header), in place on lamW / lamZ
+// pointwise part of one stage adjoint (see header) for one link: z = Z', y =
+// lambda_W' in; lz (lambda_Z, accumulated) and the returned lambda_W out
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NTF) void k_flow_stage_adj(const T *__restrict__ Zs, T *__restrict__ lamW,
-                                                        T *__restrict__ lamZ, int V, int nd,
-                                                        double c, int zero_lamZ) {
-  const int i = blockIdx.x * NTF + threadIdx.x;
-  if (i >= V * nd) return;
-  const int mu = i / V, s = i - mu * V;
-  const v3<T> z = ld3(Zs, mu, V, s);
-  const v3<T> y = ld3(lamW, mu, V, s);
-  const v3<T> a{T(c) * z.x, T(c) * z.y, T(c) * z.z};
+__device__ __forceinline__ v3<T> stage_adj_link(const v3<T> &z, const v3<T> &y, v3<T> &lz, T c) {
+  const v3<T> a{c * z.x, c * z.y, c * z.z};
   const quat<T> A = vq(a), Y = vq(y);
   // J(-A) y = y - f1 [A, y] + f2 [A, [A, y]],  theta = 2|a| (|ad_A| = 2|a|)
   const T al2 = a.x * a.x + a.y * a.y + a.z * a.z;
@@ -96,17 +91,30 @@ __global__ __launch_bounds__(NTF) void k_flow_stage_adj(const T *__restrict__ Zs
   const quat<T> k2 = comm(A, k1);
   const v3<T> jy = im(quat<T>{T(0), Y.ai - f1 * k1.ai + f2 * k2.ai, Y.br - f1 * k1.br + f2 * k2.br,
                               Y.bi - f1 * k1.bi + f2 * k2.bi});
-  v3<T> lz = zero_lamZ ? v3<T>{0, 0, 0} : ld3(lamZ, mu, V, s);
-  vadd(lz, jy, T(c));
+  vadd(lz, jy, c);
   // Ad(exp(A))^T y = g^+ y g
   const quat<T> g = qexp(a.x, a.y, a.z);
-  const v3<T> yw = im(qmul(qmul(qdag(g), Y), g));
-  lamZ[(mu * 3 + 0) * V + s] = lz.x;
-  lamZ[(mu * 3 + 1) * V + s] = lz.y;
-  lamZ[(mu * 3 + 2) * V + s] = lz.z;
-  lamW[(mu * 3 + 0) * V + s] = yw.x;
-  lamW[(mu * 3 + 1) * V + s] = yw.y;
-  lamW[(mu * 3 + 2) * V + s] = yw.z;
+  return im(qmul(qmul(qdag(g), Y), g));
+}
+
+template <typename T> __device__ __forceinline__ void st3(T *P, int plane, int V, int s, const v3<T> &v) {
+  P[(plane * 3 + 0) * V + s] = v.x;
+  P[(plane * 3 + 1) * V + s] = v.y;
+  P[(plane * 3 + 2) * V + s] = v.z;
+}
+
+// one stage adjoint, one thread per link, in place on lamW / lamZ
+template <typename T>
+__global__ __launch_bounds__(NTF) void k_flow_stage_adj(const T *__restrict__ Zs, T *__restrict__ lamW,
+                                                        T *__restrict__ lamZ, int V, int nd,
+                                                        double c, int zero_lamZ) {
+  const int i = blockIdx.x * NTF + threadIdx.x;
+  if (i >= V * nd) return;
+  const int mu = i / V, s = i - mu * V;
+  v3<T> lz = zero_lamZ ? v3<T>{0, 0, 0} : ld3(lamZ, mu, V, s);
+  const v3<T> yw = stage_adj_link<T>(ld3(Zs, mu, V, s), ld3(lamW, mu, V, s), lz, T(c));
+  st3(lamZ, mu, V, s, lz);
+  st3(lamW, mu, V, s, yw);
 }
 
 // the four role gradients of -Re(P Omega) for the HVP insertion pattern of one
@@ -262,6 +270,47 @@ __global__ __launch_bounds__(NTF) void k_flow_vjp_gather(const T *__restrict__ R
   }
 }
 
+// pass 2 of one stage's Hessian-vector product fused with the pointwise
+// adjoint of the stage before it: per site x and link (x, rho), the b/c role
+// terms are gathered as in k_flow_vjp_gather (same order, same rounding), which
+// completes lambda_W of this stage, and stage_adj_link then runs on it with the
+// earlier stage's Z' and coefficient c.  Saves one pass over lamW and lamZ.
+template <typename T, int ND>
+__global__ __launch_bounds__(NTF) void k_flow_gather_stage_adj(const T *__restrict__ R,
+                                                               T *__restrict__ lamW,
+                                                               T *__restrict__ lamZ,
+                                                               const T *__restrict__ Zs, Geo geo,
+                                                               double bcoef, double c,
+                                                               int zero_lamZ) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = lb * NTF + threadIdx.x;
+  const int V = geo.V;
+  if (s >= V) return;
+  int up[ND], dn[ND];
+  neighbour_offsets<ND>(geo, s, up, dn);
+  const T f = T(2.0 * bcoef);
+#pragma unroll
+  for (int rho = 0; rho < ND; ++rho) {
+    v3<T> g{0, 0, 0};
+#pragma unroll
+    for (int sg = 0; sg < ND; ++sg) {
+      if (sg == rho) continue;
+      const int p = rho < sg ? plane_id(rho, sg) : plane_id(sg, rho);
+      const T *Rr = R + (size_t)(p * 2 + (rho < sg ? 1 : 0)) * 3 * V;
+      const int sm = s + dn[sg];
+      vadd(g, v3<T>{Rr[sm], Rr[V + sm], Rr[2 * V + sm]});
+    }
+    v3<T> y = ld3(lamW, rho, V, s);
+    y.x += f * g.x;
+    y.y += f * g.y;
+    y.z += f * g.z;
+    v3<T> lz = zero_lamZ ? v3<T>{0, 0, 0} : ld3(lamZ, rho, V, s);
+    const v3<T> yw = stage_adj_link<T>(ld3(Zs, rho, V, s), y, lz, T(c));
+    st3(lamZ, rho, V, s, lz);
+    st3(lamW, rho, V, s, yw);
+  }
+}
+
 // pass 1 of grad E: Gamma_{mu nu}(x) = Im C_{mu nu}(x) (clover, flat-energy_density.hh:85-119)
 template <typename T, int ND>
 __global__ __launch_bounds__(NTF) void k_clover_im(const quat<T> *__restrict__ U, T *__restrict__ Gam,
@@ -354,25 +403,61 @@ void launch_flow_stage_adj(int prec, const void *Zs, void *lamW, void *lamZ, int
 
 template <typename T, int ND>
 static void vjp_nd(const void *U, const void *lamZ, void *lamW, void *R, const Geo &g, double b,
-                   double w_t, double w_s, hipStream_t st) {
+                   double w_t, double w_s, bool gather, hipStream_t st) {
   hipLaunchKernelGGL((k_flow_vjp_plaq<T, ND>), dim3(nblkf(g.V)), dim3(NTF), 0, st,
                      (const quat<T> *)U, (const T *)lamZ, (T *)lamW, (T *)R, g, b, w_t, w_s);
-  hipLaunchKernelGGL((k_flow_vjp_gather<T, ND>), dim3(nblkf(g.V)), dim3(NTF), 0, st,
-                     (const T *)R, (T *)lamW, g, b);
+  if (gather)
+    hipLaunchKernelGGL((k_flow_vjp_gather<T, ND>), dim3(nblkf(g.V)), dim3(NTF), 0, st,
+                       (const T *)R, (T *)lamW, g, b);
 }
 template <typename T>
 static void vjp_t(const void *U, const void *lamZ, void *lamW, void *R, const Geo &g, double b,
-                  double w_t, double w_s, hipStream_t st) {
+                  double w_t, double w_s, bool gather, hipStream_t st) {
   switch (g.nd) {
-    case 2: vjp_nd<T, 2>(U, lamZ, lamW, R, g, b, w_t, w_s, st); break;
-    case 3: vjp_nd<T, 3>(U, lamZ, lamW, R, g, b, w_t, w_s, st); break;
-    default: vjp_nd<T, 4>(U, lamZ, lamW, R, g, b, w_t, w_s, st); break;
+    case 2: vjp_nd<T, 2>(U, lamZ, lamW, R, g, b, w_t, w_s, gather, st); break;
+    case 3: vjp_nd<T, 3>(U, lamZ, lamW, R, g, b, w_t, w_s, gather, st); break;
+    default: vjp_nd<T, 4>(U, lamZ, lamW, R, g, b, w_t, w_s, gather, st); break;
   }
 }
 void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, void *R, const Geo &g,
-                     double b, double w_t, double w_s, hipStream_t st) {
-  if (prec == 0) vjp_t<double>(U, lamZ, lamW, R, g, b, w_t, w_s, st);
-  else vjp_t<float>(U, lamZ, lamW, R, g, b, w_t, w_s, st);
+                     double b, double w_t, double w_s, bool gather, hipStream_t st) {
+  if (prec == 0) vjp_t<double>(U, lamZ, lamW, R, g, b, w_t, w_s, gather, st);
+  else vjp_t<float>(U, lamZ, lamW, R, g, b, w_t, w_s, gather, st);
+}
+
+template <typename T>
+static void gather_t(const void *R, void *lamW, const Geo &g, double b, hipStream_t st) {
+  const dim3 grid(nblkf(g.V));
+  switch (g.nd) {
+    case 2: hipLaunchKernelGGL((k_flow_vjp_gather<T, 2>), grid, dim3(NTF), 0, st, (const T *)R, (T *)lamW, g, b); break;
+    case 3: hipLaunchKernelGGL((k_flow_vjp_gather<T, 3>), grid, dim3(NTF), 0, st, (const T *)R, (T *)lamW, g, b); break;
+    default: hipLaunchKernelGGL((k_flow_vjp_gather<T, 4>), grid, dim3(NTF), 0, st, (const T *)R, (T *)lamW, g, b); break;
+  }
+}
+void launch_flow_gather(int prec, const void *R, void *lamW, const Geo &g, double b, hipStream_t st) {
+  if (prec == 0) gather_t<double>(R, lamW, g, b, st);
+  else gather_t<float>(R, lamW, g, b, st);
+}
+
+template <typename T>
+static void gather_adj_t(const void *R, void *lamW, void *lamZ, const void *Zs, const Geo &g,
+                         double b, double c, int zero_lamZ, hipStream_t st) {
+  const dim3 grid(nblkf(g.V));
+#define SU2_GADJ(ND)                                                                            \
+  hipLaunchKernelGGL((k_flow_gather_stage_adj<T, ND>), grid, dim3(NTF), 0, st, (const T *)R,   \
+                     (T *)lamW, (T *)lamZ, (const T *)Zs, g, b, c, zero_lamZ)
+  switch (g.nd) {
+    case 2: SU2_GADJ(2); break;
+    case 3: SU2_GADJ(3); break;
+    default: SU2_GADJ(4); break;
+  }
+#undef SU2_GADJ
+}
+void launch_flow_gather_stage_adj(int prec, const void *R, void *lamW, void *lamZ, const void *Zs,
+                                  const Geo &g, double b, double c, int zero_lamZ,
+                                  hipStream_t st) {
+  if (prec == 0) gather_adj_t<double>(R, lamW, lamZ, Zs, g, b, c, zero_lamZ, st);
+  else gather_adj_t<float>(R, lamW, lamZ, Zs, g, b, c, zero_lamZ, st);
 }
 
 template <typename T>

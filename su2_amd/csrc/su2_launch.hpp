@@ -91,9 +91,18 @@ int sweep_blocks(const Geo &g);
 void launch_sweep(int group, int prec, void *U, const Geo &g, int mu, int parity, uint64_t seed,
                   uint64_t sweep, double delta, int n_hit, double beta_nc, double w_t, double w_s,
                   int aniso, double *partials, hipStream_t st);
-// R: scratch of 36 V reals (the b and c role gradients of the 6 planes)
+// lamW += b (dD/dW)^T lamZ; R: scratch of 36 V reals (the b and c role
+// gradients of the 6 planes).  gather = false leaves the R terms to a later
+// launch_flow_gather or launch_flow_gather_stage_adj.
 void launch_flow_vjp(int prec, const void *U, const void *lamZ, void *lamW, void *R, const Geo &g,
-                     double b, double w_t, double w_s, hipStream_t st);
+                     double b, double w_t, double w_s, bool gather, hipStream_t st);
+void launch_flow_gather(int prec, const void *R, void *lamW, const Geo &g, double b,
+                        hipStream_t st);
+// the gather of one stage, then the pointwise adjoint of the stage before it
+// (Z' = Zs, coefficient c) on every link
+void launch_flow_gather_stage_adj(int prec, const void *R, void *lamW, void *lamZ, const void *Zs,
+                                  const Geo &g, double b, double c, int zero_lamZ,
+                                  hipStream_t st);
 void launch_clover_grad(int prec, const void *U, void *Gam, void *lamW, const Geo &g,
                         double scale, hipStream_t st);
 
