@@ -423,15 +423,15 @@ int fused_mode(const Geo &g) {
 }
 
 // SU2GPU_FAST = 0 (generic kernels only) | 1 (specialised SU(2) d=4 step for
-// FP32, default) | 2 (also for FP64); SU2GPU_TBLOCK = 0 disables the t-blocked
-// XCD layout of the specialised step
+// the FP32 form only) | 2 (also for FP64, default); SU2GPU_TBLOCK = 0 disables
+// the t-blocked XCD layout of the specialised step
 static int env_int(const char *name, int def) {
   const char *e = getenv(name);
   return e ? atoi(e) : def;
 }
 static int fast_level() {
   static int v = -1;
-  if (v < 0) v = env_int("SU2GPU_FAST", 1);
+  if (v < 0) v = env_int("SU2GPU_FAST", 2);
   return v;
 }
 static int tblock_pref() {

@@ -484,6 +484,8 @@ def test_fp64_trajectory_parity_32(S, oracle):
         lat.md_finish(True)
         Ug = lat.get_links()
         plaq = lat.gauge_energy()
+    print(f"32^4 FP64 OMF2(20): max|dU| = {np.abs(Ug - Uw).max():.3e}, "
+          f"dH gpu {r['deltaH']:.12f} oracle {h_new - h_old:.12f}")
     assert np.abs(Ug - Uw).max() <= 1e-10
     assert r["deltaH"] == pytest.approx(h_new - h_old, rel=1e-9, abs=1e-8)
     assert plaq == pytest.approx(oracle.retr_sum_plaq(1, dims, Uw, compensated=True), rel=1e-9)
