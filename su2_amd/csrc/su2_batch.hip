@@ -3,9 +3,11 @@
 //
 // All lattices live in three link arenas and one momentum arena.  A lattice is
 // cut into tiles of `tile` sites; one MD tick (one "P then U" pair of every
-// lattice's integrator, or its final momentum step) is ONE launch of a
-// grid-stride kernel over the whole tile list, whatever the lattice's group and
-// dimension: a tile selects its (group, d) code path uniformly for the block.
+// lattice's integrator, or its final momentum step) is two launches of a
+// grid-stride kernel: one over the tiles of the SU(2) d = 4 lattices (the
+// specialised step body of su2_md_fast.hpp), one over the rest, whatever the
+// lattice's group and dimension: a tile selects its (group, d) code path
+// uniformly for the block.
 // Lattice l's k-th link update reads arena r(k) and writes w(k) with
 //   r(0) = a0, r(k>0) = (k odd ? a1 : a2), w(k) = (k even ? a1 : a2),
 //   a0 = cur_l, a1 = cur_l+1 mod 3, a2 = cur_l+2 mod 3,
@@ -13,7 +15,7 @@
 // trajectory costs nothing (cur_l is kept) while an accepted one moves cur_l to
 // the last written arena.
 #include "su2_batch.hpp"
-#include "su2_md_site.hpp"
+#include "su2_md_fast.hpp"
 
 using namespace su2dev;
 
@@ -32,6 +34,36 @@ __device__ __forceinline__ void pair_coef(int kind, int k, double &cp, double &c
 }
 __device__ __forceinline__ double final_coef(int kind) { return kind == 6 ? 0.1938 : 0.5; }
 
+// SU(2) d = 4: the specialised step body (mu selects one of four compiled
+// bodies; it is uniform over the tile), neighbour offsets from the lattice's
+// own geometry
+template <typename T, bool WRITE_U, bool ZS>
+__device__ __forceinline__ void spec_step(const BLat &L, char *const *arena, T *Parena, int rd, int wr,
+                                          const FusedCoef &c, int s, int mu, double &k0, double &a,
+                                          double &k1) {
+  const quat<T> *Uin = (const quat<T> *)(arena[rd] + L.uoff);
+  quat<T> *Uout = WRITE_U ? (quat<T> *)(arena[wr] + L.uoff) : nullptr;
+  T *P = Parena + L.poff;
+  FastGeo g{};
+  g.V = L.geo.V;
+  g.Lz = L.geo.L[3];
+  FastSite fs;
+  fs.s = s;
+  fs.mu = mu;
+  neighbour_offsets<4>(L.geo, s, fs.up, fs.dn);
+  T pi[3], pn[3];
+  double mar;
+  switch (mu) {
+    case 0: md_su2_core<T, 0, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    case 1: md_su2_core<T, 1, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    case 2: md_su2_core<T, 2, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    default: md_su2_core<T, 3, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+  }
+  k0 += sq3(pi[0], pi[1], pi[2]);
+  a += 2.0 * mar;
+  k1 += sq3(pn[0], pn[1], pn[2]);
+}
+
 template <typename T, class G, int ND, bool WRITE_U>
 __device__ __forceinline__ void tile_step(const BLat &L, char *const *arena, T *Parena, int rd, int wr,
                                           const FusedCoef &c, int s, int mu, double &k0, double &a,
@@ -43,10 +75,17 @@ __device__ __forceinline__ void tile_step(const BLat &L, char *const *arena, T *
   else md_link<G, ND, WRITE_U, false>(Uin, Uout, Parena + L.poff, L.geo, c, s, mu, k0, a, k1);
 }
 
-template <typename T, bool WRITE_U>
+// SPEC: the tile belongs to a specialised SU(2) d = 4 lattice (first part of
+// the tile list) or to any other lattice (second part)
+template <typename T, bool WRITE_U, bool SPEC>
 __device__ __forceinline__ void dispatch_step(const BLat &L, char *const *arena, T *Parena, int rd,
                                               int wr, const FusedCoef &c, int s, int mu,
                                               double &k0, double &a, double &k1) {
+  if constexpr (SPEC) {
+    if (L.zs) spec_step<T, WRITE_U, true>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1);
+    else spec_step<T, WRITE_U, false>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1);
+    return;
+  }
   const int code = L.group * 8 + L.geo.nd;
   switch (code) {
     case 0 * 8 + 2: tile_step<T, GU1<T>, 2, WRITE_U>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1); break;
@@ -60,14 +99,15 @@ __device__ __forceinline__ void dispatch_step(const BLat &L, char *const *arena,
 
 // One MD tick of every lattice.  partials rows (ntiles each): 0 |P_in|^2 and
 // 1 ReTr(U K) at a lattice's first pair, 2 |P_out|^2 at its final momentum step.
-template <typename T>
+// Tiles [t0, t0 + count) of the list; partial rows are ntiles long.
+template <typename T, bool SPEC>
 __global__ __launch_bounds__(256) void k_batch_tick(const BLat *__restrict__ lats,
-                                                    const int2 *__restrict__ tiles, int ntiles,
-                                                    BArenas ar, int tick, int flags,
-                                                    double *__restrict__ partials) {
+                                                    const int2 *__restrict__ tiles, int t0,
+                                                    int count, int ntiles, BArenas ar, int tick,
+                                                    int flags, double *__restrict__ partials) {
   const int nt = blockDim.x;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int lt = xcd_block(t, ntiles);
+  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int lt = t0 + xcd_block(t, count);
     const int2 tl = tiles[lt];
     const BLat &L = lats[tl.x];
     if (tick > L.npairs) continue;  // lattice finished (uniform over the block)
@@ -94,8 +134,8 @@ __global__ __launch_bounds__(256) void k_batch_tick(const BLat *__restrict__ lat
       wr = rd;
     }
     if (s < L.geo.V) {  // always true on z-shuffle lattices (V % tile == 0)
-      if (!final_p) dispatch_step<T, true>(L, ar.U, (T *)ar.P, rd, wr, c, s, mu, k0, act, k1);
-      else dispatch_step<T, false>(L, ar.U, (T *)ar.P, rd, wr, c, s, mu, k0, act, k1);
+      if (!final_p) dispatch_step<T, true, SPEC>(L, ar.U, (T *)ar.P, rd, wr, c, s, mu, k0, act, k1);
+      else dispatch_step<T, false, SPEC>(L, ar.U, (T *)ar.P, rd, wr, c, s, mu, k0, act, k1);
     }
     if (tick == 0 || final_p) {
       // uniform over the block: every thread of the tile takes this branch
@@ -269,12 +309,25 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BLat *__restrict__ l
 }
 
 // ------------------------------------------------------------- launchers ----
-void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
-                       int tick, int flags, double *partials, int grid, int block, hipStream_t st) {
-  if (prec == 0)
-    hipLaunchKernelGGL((k_batch_tick<double>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, tick, flags, partials);
-  else
-    hipLaunchKernelGGL((k_batch_tick<float>), dim3(grid), dim3(block), 0, st, lats, tiles, ntiles, ar, tick, flags, partials);
+void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, int nspec,
+                       const BArenas &ar, int tick, int flags, double *partials, int grid,
+                       int block, hipStream_t st) {
+  const int parts[2][2] = {{0, nspec}, {nspec, ntiles - nspec}};
+  for (int k = 0; k < 2; ++k) {
+    const int t0 = parts[k][0], n = parts[k][1];
+    if (n <= 0) continue;
+    const dim3 g(grid < n ? grid : n), bl(block);
+#define SU2_TICK(T, S) \
+  hipLaunchKernelGGL((k_batch_tick<T, S>), g, bl, 0, st, lats, tiles, t0, n, ntiles, ar, tick, flags, partials)
+    if (prec == 0) {
+      if (k == 0) SU2_TICK(double, true);
+      else SU2_TICK(double, false);
+    } else {
+      if (k == 0) SU2_TICK(float, true);
+      else SU2_TICK(float, false);
+    }
+#undef SU2_TICK
+  }
 }
 void launch_batch_plaq(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
                        int use_final, int row, int rowlen, double *partials, int grid, int block,

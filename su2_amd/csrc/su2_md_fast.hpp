@@ -286,12 +286,15 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat
 // three planes nu != mu are known at compile time and all 18 staple links are
 // loaded before any arithmetic (one memory round trip per thread).  MU < 0:
 // mu at run time, one round trip per plane.
-template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false>
-__device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
+// md_su2_core: the loads, staple sum and update; returns the input momentum
+// (pi), the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it
+// reads g.V and, with ZS, g.Lz.
+template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false>
+__device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
-                                            const FastSite &fs, double *__restrict__ partials,
-                                            int pstride, double *lds) {
+                                            const FastSite &fs, T (&pi)[3], T (&pn)[3],
+                                            double &mar) {
   const int mu = MU >= 0 ? MU : fs.mu;
   const int s = fs.s;
   const int *up = fs.up, *dn = fs.dn;
@@ -370,11 +373,24 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
                   ldo(Um, (uint32_t)(s + dn[nu])), ldo(Un, (uint32_t)(s + dn[nu])));
     }
   }
-  T pn[3];
-  const double mar = md_su2_update<T, WRITE_U>(u, K, pi0, pi1, pi2, P0, P1, P2,
-                                               Uout + (size_t)mu * V, (uint32_t)s, c, pn);
+  pi[0] = pi0;
+  pi[1] = pi1;
+  pi[2] = pi2;
+  mar = md_su2_update<T, WRITE_U>(u, K, pi0, pi1, pi2, P0, P1, P2, Uout + (size_t)mu * V,
+                                  (uint32_t)s, c, pn);
+}
+
+template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false>
+__device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
+                                            quat<T> *__restrict__ Uout, T *__restrict__ P,
+                                            const FastGeo &g, const FusedCoef &c,
+                                            const FastSite &fs, double *__restrict__ partials,
+                                            int pstride, double *lds) {
+  T pi[3], pn[3];
+  double mar;
+  md_su2_core<T, MU, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar);
   if (ACCUM) {
-    const double r0 = block_sum<NTF>(sq3(pi0, pi1, pi2), lds);
+    const double r0 = block_sum<NTF>(sq3(pi[0], pi[1], pi[2]), lds);
     const double r1 = block_sum<NTF>(2.0 * mar, lds);
     const double r2 = block_sum<NTF>(sq3(pn[0], pn[1], pn[2]), lds);
     if (threadIdx.x == 0) {
