@@ -65,7 +65,8 @@ struct su2gpu_batch {
   su2k::BArenas ar{};
   su2k::BLat *dlat = nullptr;
   int2 *dtiles = nullptr, *dltiles = nullptr;
-  int ntiles = 0, nltiles = 0, nspec = 0, block = 256, grid = 0, grid_site = 0, maxpairs = 0;
+  int ntiles = 0, nltiles = 0, nspec4 = 0, nspec3 = 0, block = 256, grid = 0, grid_site = 0,
+      maxpairs = 0;
   double *partials = nullptr, *dsums = nullptr, *hsums = nullptr;
   uint64_t *dtraj = nullptr, *htraj = nullptr;
   double *staging = nullptr, *hstage = nullptr;
@@ -86,17 +87,17 @@ int npairs_of(int kind, size_t n) { return kind == SU2GPU_OMF2 ? (int)(2 * n) : 
 
 // site tiles (tile size = block) and link tiles (one per site tile and mu, mu
 // fastest, site start packed as 4*s + mu); lattices contiguous
-// The link tiles of the SU(2) d = 4 lattices (the specialised step body) come
-// first, [0, nspec), so that each MD tick is two launches with their own
-// register budgets: the specialised one over them, the generic one over the rest.
+// The link tiles of the SU(2) lattices in d = 4, then in d = 3 (the specialised
+// step body) come first, so that each MD tick is three launches with their own
+// register budgets: one per specialised dimension and the generic one.
 int build_tiles(B *b) {
   std::vector<int2> tiles, ltiles;
-  // SU2GPU_BATCH_SPEC = 0 keeps SU(2) d = 4 lattices on the generic step
+  // SU2GPU_BATCH_SPEC = 0 keeps the SU(2) d = 3, 4 lattices on the generic step
   const char *e = std::getenv("SU2GPU_BATCH_SPEC");
   const bool spec_on = !(e && std::atoi(e) == 0);
   for (auto &L : b->lat) {
     L.zs = (64 % L.geo.L[L.geo.nd - 1] == 0) && (L.geo.V % b->block == 0);
-    L.spec = (spec_on && L.group == 1 && L.geo.nd == 4) ? 1 : 0;
+    L.spec = (spec_on && L.group == 1 && L.geo.nd >= 3) ? L.geo.nd : 0;
   }
   for (size_t l = 0; l < b->lat.size(); ++l) {
     auto &L = b->lat[l];
@@ -104,7 +105,8 @@ int build_tiles(B *b) {
     for (int s = 0; s < L.geo.V; s += b->block) tiles.push_back(int2{(int)l, s});
     L.ntiles = (int)tiles.size() - L.tile0;
   }
-  for (int pass = 1; pass >= 0; --pass) {
+  for (int pass : {4, 3, 0}) {
+    const int before = (int)ltiles.size();
     for (size_t l = 0; l < b->lat.size(); ++l) {
       auto &L = b->lat[l];
       if (L.spec != pass) continue;
@@ -113,7 +115,8 @@ int build_tiles(B *b) {
         for (int mu = 0; mu < L.geo.nd; ++mu) ltiles.push_back(int2{(int)l, 4 * s + mu});
       L.nltiles = (int)ltiles.size() - L.ltile0;
     }
-    if (pass == 1) b->nspec = (int)ltiles.size();
+    if (pass == 4) b->nspec4 = (int)ltiles.size() - before;
+    if (pass == 3) b->nspec3 = (int)ltiles.size() - before;
   }
   b->ntiles = (int)tiles.size();
   b->nltiles = (int)ltiles.size();
@@ -164,8 +167,8 @@ int enqueue(B *b, bool devmom) {
                           b->partials, b->grid_site, b->block, b->stream);
   const int flags = (b->prec == 1) ? (su2k::F_COMPENSATED | su2k::F_RESTORE) : 0;
   for (int t = 0; t <= b->maxpairs; ++t)
-    su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec, b->ar, t, flags,
-                            b->partials, b->grid, b->block, b->stream);
+    su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3, b->ar,
+                            t, flags, b->partials, b->grid, b->block, b->stream);
   if (b->prec == 0)
     su2k::launch_batch_restore(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->grid_site,
                                b->block, b->stream);

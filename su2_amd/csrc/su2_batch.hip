@@ -3,9 +3,9 @@
 //
 // All lattices live in three link arenas and one momentum arena.  A lattice is
 // cut into tiles of `tile` sites; one MD tick (one "P then U" pair of every
-// lattice's integrator, or its final momentum step) is two launches of a
-// grid-stride kernel: one over the tiles of the SU(2) d = 4 lattices (the
-// specialised step body of su2_md_fast.hpp), one over the rest, whatever the
+// lattice's integrator, or its final momentum step) is three launches of a
+// grid-stride kernel: over the tiles of the SU(2) lattices in d = 4, in d = 3
+// (the specialised step body of su2_md_fast.hpp), and over the rest, whatever the
 // lattice's group and dimension: a tile selects its (group, d) code path
 // uniformly for the block.
 // Lattice l's k-th link update reads arena r(k) and writes w(k) with
@@ -34,10 +34,10 @@ __device__ __forceinline__ void pair_coef(int kind, int k, double &cp, double &c
 }
 __device__ __forceinline__ double final_coef(int kind) { return kind == 6 ? 0.1938 : 0.5; }
 
-// SU(2) d = 4: the specialised step body (mu selects one of four compiled
+// SU(2) in d = 3, 4: the specialised step body (mu selects one of ND compiled
 // bodies; it is uniform over the tile), neighbour offsets from the lattice's
 // own geometry
-template <typename T, bool WRITE_U, bool ZS>
+template <typename T, int ND, bool WRITE_U, bool ZS>
 __device__ __forceinline__ void spec_step(const BLat &L, char *const *arena, T *Parena, int rd, int wr,
                                           const FusedCoef &c, int s, int mu, double &k0, double &a,
                                           double &k1) {
@@ -46,18 +46,20 @@ __device__ __forceinline__ void spec_step(const BLat &L, char *const *arena, T *
   T *P = Parena + L.poff;
   FastGeo g{};
   g.V = L.geo.V;
-  g.Lz = L.geo.L[3];
-  FastSite fs;
+  g.Lz = L.geo.L[ND - 1];
+  FastSite fs{};
   fs.s = s;
   fs.mu = mu;
-  neighbour_offsets<4>(L.geo, s, fs.up, fs.dn);
+  neighbour_offsets<ND>(L.geo, s, fs.up, fs.dn);
   T pi[3], pn[3];
   double mar;
   switch (mu) {
-    case 0: md_su2_core<T, 0, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
-    case 1: md_su2_core<T, 1, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
-    case 2: md_su2_core<T, 2, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
-    default: md_su2_core<T, 3, WRITE_U, ZS>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    case 0: md_su2_core<T, 0, WRITE_U, ZS, false, ND>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    case 1: md_su2_core<T, 1, WRITE_U, ZS, false, ND>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    case 2: md_su2_core<T, 2, WRITE_U, ZS, false, ND>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    default:
+      if constexpr (ND == 4) md_su2_core<T, 3, WRITE_U, ZS, false, ND>(Uin, Uout, P, g, c, fs, pi, pn, mar);
+      break;
   }
   k0 += sq3(pi[0], pi[1], pi[2]);
   a += 2.0 * mar;
@@ -75,15 +77,15 @@ __device__ __forceinline__ void tile_step(const BLat &L, char *const *arena, T *
   else md_link<G, ND, WRITE_U, false>(Uin, Uout, Parena + L.poff, L.geo, c, s, mu, k0, a, k1);
 }
 
-// SPEC: the tile belongs to a specialised SU(2) d = 4 lattice (first part of
-// the tile list) or to any other lattice (second part)
-template <typename T, bool WRITE_U, bool SPEC>
+// SPEC = 4, 3: the tile belongs to a specialised SU(2) lattice in that
+// dimension (first and second part of the tile list); 0: any other lattice
+template <typename T, bool WRITE_U, int SPEC>
 __device__ __forceinline__ void dispatch_step(const BLat &L, char *const *arena, T *Parena, int rd,
                                               int wr, const FusedCoef &c, int s, int mu,
                                               double &k0, double &a, double &k1) {
-  if constexpr (SPEC) {
-    if (L.zs) spec_step<T, WRITE_U, true>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1);
-    else spec_step<T, WRITE_U, false>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1);
+  if constexpr (SPEC != 0) {
+    if (L.zs) spec_step<T, SPEC, WRITE_U, true>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1);
+    else spec_step<T, SPEC, WRITE_U, false>(L, arena, Parena, rd, wr, c, s, mu, k0, a, k1);
     return;
   }
   const int code = L.group * 8 + L.geo.nd;
@@ -100,7 +102,7 @@ __device__ __forceinline__ void dispatch_step(const BLat &L, char *const *arena,
 // One MD tick of every lattice.  partials rows (ntiles each): 0 |P_in|^2 and
 // 1 ReTr(U K) at a lattice's first pair, 2 |P_out|^2 at its final momentum step.
 // Tiles [t0, t0 + count) of the list; partial rows are ntiles long.
-template <typename T, bool SPEC>
+template <typename T, int SPEC>
 __global__ __launch_bounds__(256) void k_batch_tick(const BLat *__restrict__ lats,
                                                     const int2 *__restrict__ tiles, int t0,
                                                     int count, int ntiles, BArenas ar, int tick,
@@ -309,23 +311,26 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BLat *__restrict__ l
 }
 
 // ------------------------------------------------------------- launchers ----
-void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, int nspec,
+void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, int n4, int n3,
                        const BArenas &ar, int tick, int flags, double *partials, int grid,
                        int block, hipStream_t st) {
-  const int parts[2][2] = {{0, nspec}, {nspec, ntiles - nspec}};
-  for (int k = 0; k < 2; ++k) {
+  const int parts[3][2] = {{0, n4}, {n4, n3}, {n4 + n3, ntiles - n4 - n3}};
+  for (int k = 0; k < 3; ++k) {
     const int t0 = parts[k][0], n = parts[k][1];
     if (n <= 0) continue;
     const dim3 g(grid < n ? grid : n), bl(block);
 #define SU2_TICK(T, S) \
   hipLaunchKernelGGL((k_batch_tick<T, S>), g, bl, 0, st, lats, tiles, t0, n, ntiles, ar, tick, flags, partials)
+#define SU2_TICK3(T)            \
+  if (k == 0) SU2_TICK(T, 4);   \
+  else if (k == 1) SU2_TICK(T, 3); \
+  else SU2_TICK(T, 0)
     if (prec == 0) {
-      if (k == 0) SU2_TICK(double, true);
-      else SU2_TICK(double, false);
+      SU2_TICK3(double);
     } else {
-      if (k == 0) SU2_TICK(float, true);
-      else SU2_TICK(float, false);
+      SU2_TICK3(float);
     }
+#undef SU2_TICK3
 #undef SU2_TICK
   }
 }

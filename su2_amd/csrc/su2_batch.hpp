@@ -18,7 +18,7 @@ struct BLat {
   int tile0, ntiles;    // site tiles (sums, momenta, restore)
   int ltile0, nltiles;  // link tiles (MD ticks): one per (site tile, mu)
   int zs;               // z-shuffle path allowed (64 % L_fastest == 0, V % tile == 0)
-  int spec;             // SU(2) d = 4 through the specialised step body (md_su2_core)
+  int spec;             // 4, 3: SU(2) in that dimension through the specialised step body (md_su2_core); 0: generic
   long uoff;      // byte offset of the lattice's links in every link arena
   long poff;      // element offset of its momenta in the momentum arena
   double coef;    // beta / Nc
@@ -31,10 +31,11 @@ struct BArenas {
   void *P;
 };
 
-// ltiles: (lattice, site_start * 4 + mu), those of the specialised SU(2) d = 4
-// lattices first ([0, nspec)); partial rows are nltiles long.  One launch per
-// part (at most `grid` blocks each).
-void launch_batch_tick(int prec, const BLat *lats, const int2 *ltiles, int nltiles, int nspec,
+// ltiles: (lattice, site_start * 4 + mu): those of the specialised SU(2)
+// lattices in d = 4 ([0, n4)), then in d = 3 ([n4, n4 + n3)), then the rest;
+// partial rows are nltiles long.  One launch per part (at most `grid` blocks
+// each), so that each part has its own register budget.
+void launch_batch_tick(int prec, const BLat *lats, const int2 *ltiles, int nltiles, int n4, int n3,
                        const BArenas &ar, int tick, int flags, double *partials, int grid,
                        int block, hipStream_t st);
 void launch_batch_plaq(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,

@@ -286,10 +286,11 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat
 // three planes nu != mu are known at compile time and all 18 staple links are
 // loaded before any arithmetic (one memory round trip per thread).  MU < 0:
 // mu at run time, one round trip per plane.
-// md_su2_core: the loads, staple sum and update; returns the input momentum
-// (pi), the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it
-// reads g.V and, with ZS, g.Lz.
-template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false>
+// md_su2_core: the loads, staple sum and update of one link in ND = 3 or 4
+// dimensions (direction ND - 1 the fastest); returns the input momentum (pi),
+// the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it reads
+// g.V and, with ZS, g.Lz (the extent of direction ND - 1).
+template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false, int ND = 4>
 __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
@@ -309,7 +310,7 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
   // so the first staple's addition folds away
   double K[4] = {-0.0, -0.0, -0.0, -0.0};
   if constexpr (MU >= 0) {
-    quat<T> A[3], B[3], C[3], D[3], E[3], F[3];
+    quat<T> A[ND - 1], B[ND - 1], C[ND - 1], D[ND - 1], E[ND - 1], F[ND - 1];
     // ZS: the +-z neighbours of the wave's sites sit in the neighbouring lanes
     // (the wave holds whole z-rows), so they are exchanged through ds_bpermute
     // instead of loaded: 4 of the 6 loads of the plane nu = z, 2 of 6 in every
@@ -321,13 +322,13 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
       zdn = (zl == 0) ? lane + (g.Lz - 1) : lane - 1;
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < ND - 1; ++k) {
       const int nu = k + (k >= MU ? 1 : 0);
       const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
-      if (ZS && nu == 3) {
+      if (ZS && nu == ND - 1) {
         A[k] = ldo(Un, (uint32_t)(s + up[MU]));         // U_z(s+mu)
         C[k] = ldo(Un, (uint32_t)s);                    // U_z(s)
-      } else if (ZS && MU == 3) {
+      } else if (ZS && MU == ND - 1) {
         B[k] = ldo(Um, (uint32_t)(s + up[nu]));         // U_z(s+nu)
         C[k] = ldo(Un, (uint32_t)s);                    // U_nu(s)
         E[k] = ldo(Um, (uint32_t)(s + dn[nu]));         // U_z(s-nu)
@@ -343,28 +344,28 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
     }
     if (ZS) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
+      for (int k = 0; k < ND - 1; ++k) {
         const int nu = k + (k >= MU ? 1 : 0);
-        if (nu == 3) {
+        if (nu == ND - 1) {
           B[k] = qshfl(u, zup);     // U_mu(s+z)
           E[k] = qshfl(u, zdn);     // U_mu(s-z)
           D[k] = qshfl(A[k], zdn);  // U_z(s+mu-z)
           F[k] = qshfl(C[k], zdn);  // U_z(s-z)
-        } else if (MU == 3) {
+        } else if (MU == ND - 1) {
           A[k] = qshfl(C[k], zup);  // U_nu(s+z)
           D[k] = qshfl(F[k], zup);  // U_nu(s+z-nu)
         }
       }
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < ND - 1; ++k) {
       const int nu = k + (k >= MU ? 1 : 0);
       staple_pair<T, AN>(K, (MU == 0 || nu == 0) ? c.w_t : c.w_s, A[k], B[k], C[k], D[k], E[k],
                          F[k]);
     }
   } else {
 #pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
+    for (int nu = 0; nu < ND; ++nu) {
       if (nu == mu) continue;
       const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
       staple_pair<T, AN>(K, (mu == 0 || nu == 0) ? c.w_t : c.w_s, ldo(Un, (uint32_t)(s + up[mu])),
