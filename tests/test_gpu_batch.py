@@ -19,6 +19,8 @@ MIXED = [
     dict(group=0, dims=(17, 1, 1, 19, 2), beta=4.0, integrator="omf2", nsteps=5),
     dict(group=1, dims=(6, 1, 1, 5, 2), beta=1.5, integrator="leapfrog", nsteps=4),
     dict(group=1, dims=(3, 4, 5, 6, 4), beta=2.3, integrator="omf2", nsteps=6),
+    dict(group=1, dims=(8, 8, 1, 8, 3), beta=4.0, integrator="leapfrog", nsteps=5),
+    dict(group=1, dims=(8, 8, 8, 8, 4), beta=2.5, integrator="omf2", nsteps=4),
 ]
 
 
