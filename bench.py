@@ -103,7 +103,7 @@ def run_form(S, prec, L, nsteps, steps, warmup, seed, beta):
     }
 
 
-def delta_dH(S, L, nsteps, beta, seed, ntraj, ntherm=50, nchain=40):
+def delta_dH(S, L, nsteps, beta, seed, ntraj, ntherm=50, nchain=100):
     """FP32 form against the FP64 form at equilibrium.
 
     Start: Haar hot start + `ntherm` FP64 OMF2 trajectories (the relaxation
@@ -316,7 +316,7 @@ def main():
     ap.add_argument("--seed", type=int, default=13526463)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ddh-traj", type=int, default=8, help="paired FP32/FP64 trajectories")
-    ap.add_argument("--chain", type=int, default=40, help="chain length of the acceptance comparison")
+    ap.add_argument("--chain", type=int, default=100, help="chain length of the acceptance comparison")
     ap.add_argument("--prec-L", type=int, default=48, help="lattice extent of the FP32-vs-FP64 study")
     ap.add_argument("--prec-beta", type=float, default=2.5, help="beta of the FP32-vs-FP64 study")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
