@@ -213,8 +213,10 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad);
  * the md_update of md_update.hh:18-81 advanced together) ---------------------------
  * A batch holds lattices of uneven sizes (U(1) / SU(2), d = 2, 3, 4, own beta,
  * integrator leapfrog or omf2, own step count and tau).  One trajectory of the
- * batch is one launch per MD tick for all lattices (optionally replayed from a
- * captured hipGraph), FP64 action / kinetic sums per lattice, and a Metropolis
+ * batch is one MD tick after the other for all lattices at once (per tick one
+ * launch for the SU(2) lattices of each of d = 4 and d = 3 and one for the
+ * rest; optionally replayed from a captured hipGraph), FP64 action / kinetic
+ * sums per lattice, and a Metropolis
  * decision per lattice with the Philox uniform of (seed, traj).  Links start cold. */
 typedef struct su2gpu_batch su2gpu_batch;
 typedef struct {
