@@ -97,13 +97,13 @@ __global__ __launch_bounds__(NTS) void k_sweep(typename G::link *__restrict__ U,
       const double v = u01_32(c[3]);
       if constexpr (G::NCOMP == 4) {
         // random_element (random_element.hh:9-25): (cos a + i sin a u, sin a r (sin th + i cos th))
-        const double alpha = delta * (2.0 * M_PI) * u01_32(c[0]);
+        // alpha = 2 pi delta u0 and theta = 2 pi u2 through sincospi (no
+        // argument reduction by pi)
         const double uu = -1.0 + 2.0 * u01_32(c[1]);
-        const double th = (2.0 * M_PI) * u01_32(c[2]);
         const double r = sqrt(1.0 - uu * uu);
         double sa, ca, st, ct;
-        sincos(alpha, &sa, &ca);
-        sincos(th, &st, &ct);
+        sincospi(2.0 * delta * u01_32(c[0]), &sa, &ca);
+        sincospi(2.0 * u01_32(c[2]), &st, &ct);
         const quat<double> R{ca, sa * uu, sa * r * st, sa * r * ct};
         const quat<double> ud{double(u.ar), double(u.ai), double(u.br), double(u.bi)};
         const quat<double> Kd{double(K.ar), double(K.ai), double(K.br), double(K.bi)};
@@ -111,8 +111,9 @@ __global__ __launch_bounds__(NTS) void k_sweep(typename G::link *__restrict__ U,
         const double dS = beta_nc * (retr_mul(ud, Kd) - retr_mul(uR, Kd));
         if (dS < 0.0 || v < exp(-dS)) {
           // restoreSU (su2.hh:62-66) in FP64, one rounding to the storage precision
-          const double nrm = sqrt(uR.ar * uR.ar + uR.ai * uR.ai + uR.br * uR.br + uR.bi * uR.bi);
-          u = link{T(uR.ar / nrm), T(uR.ai / nrm), T(uR.br / nrm), T(uR.bi / nrm)};
+          const double rn =
+              1.0 / sqrt(uR.ar * uR.ar + uR.ai * uR.ai + uR.br * uR.br + uR.bi * uR.bi);
+          u = link{T(uR.ar * rn), T(uR.ai * rn), T(uR.br * rn), T(uR.bi * rn)};
           accepted += 1.0;
         }
       } else {
