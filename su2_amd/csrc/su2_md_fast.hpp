@@ -542,6 +542,51 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_site(const quat<T> *__restric
   md_su2_site_body<T, WRITE_U, ACCUM>(Uin, Uout, P, g, c, partials, pstride, lds);
 }
 
+// Any other geometry (FP32 form): the same step body with the neighbour offsets
+// of the lattice's own geometry, for lattices whose 3-volume is not a multiple of NTF
+// (no launch-layout coordinates, no t-blocking).  Blocks of NTF links of one
+// mu, nb = ceil(V / NTF) blocks per mu in XCD-aware order (the partial-sum
+// count 4 nb of the generic thread-per-link kernel); lanes beyond V idle.
+// ZS needs whole waves (V % 64 == 0) and 64 % Lz == 0.
+template <typename T, bool WRITE_U, bool ACCUM, bool ZS, bool AN>
+__global__ __launch_bounds__(NTF) void k_md_su2_geo(const quat<T> *__restrict__ Uin,
+                                                  quat<T> *__restrict__ Uout,
+                                                  T *__restrict__ P, Geo geo, FusedCoef c,
+                                                  double *__restrict__ partials, int pstride) {
+  __shared__ double lds[NTF / 64];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int nb = (geo.V + NTF - 1) / NTF;
+  const int mu = lb / nb;
+  const int s = (lb - mu * nb) * NTF + threadIdx.x;
+  T pi[3] = {T(0), T(0), T(0)}, pn[3] = {T(0), T(0), T(0)};
+  double mar = 0.0;
+  if (s < geo.V) {
+    FastGeo g{};
+    g.V = geo.V;
+    g.Lz = geo.L[3];
+    FastSite fs{};
+    fs.s = s;
+    fs.mu = mu;
+    neighbour_offsets<4>(geo, s, fs.up, fs.dn);
+    switch (mu) {
+      case 0: md_su2_core<T, 0, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+      case 1: md_su2_core<T, 1, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+      case 2: md_su2_core<T, 2, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+      default: md_su2_core<T, 3, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar); break;
+    }
+  }
+  if (ACCUM) {
+    const double r0 = block_sum<NTF>(sq3(pi[0], pi[1], pi[2]), lds);
+    const double r1 = block_sum<NTF>(2.0 * mar, lds);
+    const double r2 = block_sum<NTF>(sq3(pn[0], pn[1], pn[2]), lds);
+    if (threadIdx.x == 0) {
+      partials[0 * pstride + blockIdx.x] = r0;
+      partials[1 * pstride + blockIdx.x] = r1;
+      partials[2 * pstride + blockIdx.x] = r2;
+    }
+  }
+}
+
 // Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) for SU(2)
 // in d = 4, one thread per site on the site layout of fast_site_s (V / NTF
 // blocks), FP64 whatever the storage precision:
