@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 study")
     ap.add_argument("--no-metropolis", action="store_true", help="skip the Metropolis-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--scan", type=str, default="",
+                    help="comma-separated extents L: also time both forms on L^4 (scan entry)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -341,6 +343,16 @@ def main():
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
     flow = None if args.no_flow else flow_adjoint(S)
     metro = None if args.no_metropolis else metropolis(S, args.L)
+    scan = []
+    for Ls in filter(None, args.scan.split(",")):
+        L = int(Ls)
+        e = {"lattice": [L] * 4}
+        for name, prec in (("fp32", S.FP32), ("fp64", S.FP64)):
+            r = run_form(S, prec, L, args.nsteps, 3, 1, args.seed, args.beta)
+            e[name] = {"link_updates_per_s": r["link_updates_per_s"], "ms_per_traj": r["ms_per_traj"],
+                       "fused_avg_us": r["fused_avg_us"],
+                       "fused_link_updates_per_s": 4 * L ** 4 / (r["fused_avg_us"] * 1e-6)}
+        scan.append(e)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
@@ -382,6 +394,8 @@ def main():
         "flow_adjoint": flow,
         "metropolis": metro,
     }
+    if scan:
+        out["scan"] = scan
     print(json.dumps(out), flush=True)
 
 
