@@ -466,6 +466,32 @@ def test_fp32_specialised_step_anisotropic(S, oracle, xi):
     assert abs(r["deltaH"] - (h1 - h0)) < 2e-3
 
 
+@pytest.mark.parametrize("dims,xi", [
+    ((5, 4, 16, 2, 4), 1.0),   # 3-volume 320, V = 640: lane exchange, last block half full
+    ((3, 5, 6, 7, 4), 1.0),    # V = 630: no lane exchange, ragged last block
+    ((5, 4, 16, 2, 4), 1.3),   # anisotropic weights
+])
+def test_fp32_step_other_geometries(S, oracle, dims, xi):
+    """The FP32 step on lattices whose 3-volume is not a multiple of 256 (the
+    specialised body over the lattice's own geometry) against the oracle's FP64
+    trajectory from the same links and host momenta."""
+    assert (dims[0] * dims[1] * dims[2]) % 256 != 0
+    U = oracle.hotstart(1, dims, SEED + 21, 0.3)
+    P = oracle.initnormal(1, dims, SEED + 22)
+    Uw, Pw = oracle.integrate(1, dims, U, P, 2.4, 6, 5, xi=xi)
+    h0 = oracle.action(1, dims, U, 2.4, xi=xi, compensated=True) + oracle.kinetic(1, dims, P, compensated=True)
+    h1 = oracle.action(1, dims, Uw, 2.4, xi=xi, compensated=True) + oracle.kinetic(1, dims, Pw, compensated=True)
+    with S.Lattice(S.SU2, dims, 2.4, xi=xi, precision=S.FP32) as lat:
+        lat.set_links(U)
+        r = lat.md_trajectory(P, 6, 5)
+        lat.md_finish(True)
+        Ug = lat.get_links()
+    assert np.abs(Ug - Uw).max() < 2e-5
+    assert abs(r["deltaH"] - (h1 - h0)) < 2e-3
+    q = Ug.reshape(-1, 4)
+    assert np.abs((q ** 2).sum(1) - 1.0).max() < 5e-7
+
+
 def test_fp64_trajectory_parity_32(S, oracle):
     """The plain FP64 form at the benchmark size: one OMF2(20) trajectory on
     32^4 at beta = 2.4 from the seeded hot start with the reference's host
