@@ -298,6 +298,35 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+// three wave64 + workgroup sums at once (one LDS round, two barriers); results
+// valid in thread 0.  lds: 3 NT / 64 doubles.
+template <int NT>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+    c += __shfl_down(c, o, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    lds[w] = a;
+    lds[NT / 64 + w] = b;
+    lds[2 * (NT / 64) + w] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = b = c = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+      a += lds[i];
+      b += lds[NT / 64 + i];
+      c += lds[2 * (NT / 64) + i];
+    }
+  }
+}
+
 // wave64 + workgroup sum of a double, result valid in thread 0
 template <int NT> __device__ __forceinline__ double block_sum(double v, double *lds) {
 #pragma unroll

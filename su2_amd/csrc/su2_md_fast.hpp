@@ -391,9 +391,8 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
   double mar;
   md_su2_core<T, MU, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar);
   if (ACCUM) {
-    const double r0 = block_sum<NTF>(sq3(pi[0], pi[1], pi[2]), lds);
-    const double r1 = block_sum<NTF>(2.0 * mar, lds);
-    const double r2 = block_sum<NTF>(sq3(pn[0], pn[1], pn[2]), lds);
+    double r0 = sq3(pi[0], pi[1], pi[2]), r1 = 2.0 * mar, r2 = sq3(pn[0], pn[1], pn[2]);
+    block_sum3<NTF>(r0, r1, r2, lds);
     if (threadIdx.x == 0) {
       partials[0 * pstride + blockIdx.x] = r0;
       partials[1 * pstride + blockIdx.x] = r1;
@@ -411,7 +410,7 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ U
                                                  quat<T> *__restrict__ Uout,
                                                  T *__restrict__ P, FastGeo g, FusedCoef c,
                                                  double *__restrict__ partials, int pstride) {
-  __shared__ double lds[NTF / 64];
+  __shared__ double lds[3 * NTF / 64];
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
   if (SPEC) {
     switch (fs.mu) {
@@ -521,9 +520,8 @@ __device__ __forceinline__ void md_su2_site_body(const quat<T> *__restrict__ Uin
     }
   }
   if (ACCUM) {
-    const double r0 = block_sum<NTF>(ko, lds);
-    const double r1 = block_sum<NTF>(act, lds);
-    const double r2 = block_sum<NTF>(kn, lds);
+    double r0 = ko, r1 = act, r2 = kn;
+    block_sum3<NTF>(r0, r1, r2, lds);
     if (threadIdx.x < 4) {
       const uint32_t e = 4u * blockIdx.x + threadIdx.x;
       partials[0 * pstride + e] = threadIdx.x == 0 ? r0 : 0.0;
@@ -538,7 +536,7 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_site(const quat<T> *__restric
                                                       quat<T> *__restrict__ Uout,
                                                       T *__restrict__ P, FastGeo g, FusedCoef c,
                                                       double *__restrict__ partials, int pstride) {
-  __shared__ double lds[NTF / 64];
+  __shared__ double lds[3 * NTF / 64];
   md_su2_site_body<T, WRITE_U, ACCUM>(Uin, Uout, P, g, c, partials, pstride, lds);
 }
 
@@ -553,7 +551,7 @@ __global__ __launch_bounds__(NTF) void k_md_su2_geo(const quat<T> *__restrict__ 
                                                   quat<T> *__restrict__ Uout,
                                                   T *__restrict__ P, Geo geo, FusedCoef c,
                                                   double *__restrict__ partials, int pstride) {
-  __shared__ double lds[NTF / 64];
+  __shared__ double lds[3 * NTF / 64];
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int nb = (geo.V + NTF - 1) / NTF;
   const int mu = lb / nb;
@@ -576,9 +574,8 @@ __global__ __launch_bounds__(NTF) void k_md_su2_geo(const quat<T> *__restrict__ 
     }
   }
   if (ACCUM) {
-    const double r0 = block_sum<NTF>(sq3(pi[0], pi[1], pi[2]), lds);
-    const double r1 = block_sum<NTF>(2.0 * mar, lds);
-    const double r2 = block_sum<NTF>(sq3(pn[0], pn[1], pn[2]), lds);
+    double r0 = sq3(pi[0], pi[1], pi[2]), r1 = 2.0 * mar, r2 = sq3(pn[0], pn[1], pn[2]);
+    block_sum3<NTF>(r0, r1, r2, lds);
     if (threadIdx.x == 0) {
       partials[0 * pstride + blockIdx.x] = r0;
       partials[1 * pstride + blockIdx.x] = r1;
@@ -601,7 +598,7 @@ __global__ __launch_bounds__(NTF) void k_plaq_su2_4d(const quat<T> *__restrict__
                                                    quat<T> *__restrict__ Uout, FastGeo g,
                                                    double w_t, double w_s, int mu_start,
                                                    int aniso, double *__restrict__ partials) {
-  __shared__ double lds[NTF / 64];
+  __shared__ double lds[3 * NTF / 64];
   int s, up[4], dn[4];
   fast_site_s(g, blockIdx.x, gridDim.x, threadIdx.x, s, up, dn);
   const int V = g.V;
