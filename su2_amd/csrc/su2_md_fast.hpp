@@ -67,7 +67,9 @@ struct FastGeo {
   int nb3;        // vol3 / NTF
   int Tc;         // Lt / 8 (tblock)
   int tblock;
-  uint32_t m_plane, m_Lz, m_Tc, m_nb3;  // multiply-high divisors (fast_div)
+  int xwalk;      // tblock walk: 3-volume blocks x-fastest (bx blocks per x plane)
+  int bx;         // plane / NTF
+  uint32_t m_plane, m_Lz, m_Tc, m_nb3, m_Lx;  // multiply-high divisors (fast_div)
 };
 
 // floor(n / d) = umulhi(n, m) with m = floor(2^32 / d) + 1, exact whenever
@@ -101,6 +103,11 @@ __host__ __device__ __forceinline__ FastSite fast_site(const FastGeo &g, uint32_
     const uint32_t r = local >> 2;
     const uint32_t q = fast_div(r, g.m_Tc);  // r = q Tc + tt
     blk3 = (int)q;
+    if (g.xwalk) {
+      // q = w Lx + x: consecutive q are x neighbours (block w of the x plane)
+      const uint32_t w = fast_div(q, g.m_Lx);
+      blk3 = (int)((q - w * (uint32_t)g.Lx) * (uint32_t)g.bx + w);
+    }
     t = (int)xcd * g.Tc + (int)(r - q * (uint32_t)g.Tc);
   } else {
     // each XCD one contiguous eighth of the logical blocks (xcd_block)
@@ -133,7 +140,8 @@ __host__ __device__ __forceinline__ FastSite fast_site(const FastGeo &g, uint32_
 // false where it does not apply: the 3-volume a multiple of the block size,
 // 32-bit byte offsets within one direction plane, and every multiply-high
 // division within its exact range (n d < 2^32)
-inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f) {
+inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f,
+                          bool allow_xwalk = false) {
   f.Lt = L[0];
   f.Lx = L[1];
   f.Ly = L[2];
@@ -154,6 +162,9 @@ inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f) 
   f.m_Lz = fast_div_magic(f.Lz);
   f.m_Tc = fast_div_magic(f.Tc);
   f.m_nb3 = fast_div_magic(f.nb3);
+  f.xwalk = (allow_xwalk && f.tblock && f.plane % NTF == 0) ? 1 : 0;
+  f.bx = f.plane / NTF;
+  f.m_Lx = fast_div_magic(f.Lx);
   return true;
 }
 

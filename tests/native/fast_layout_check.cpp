@@ -10,10 +10,10 @@
 
 using namespace su2k;
 
-static int check(const int L[4], bool tblock) {
+static int check(const int L[4], bool tblock, bool xwalk) {
   const int V = L[0] * L[1] * L[2] * L[3];
   FastGeo g;
-  if (!make_fast_geo(L, V, tblock, g)) {
+  if (!make_fast_geo(L, V, tblock, g, xwalk)) {
     printf("%dx%dx%dx%d tblock=%d: not applicable\n", L[0], L[1], L[2], L[3], (int)tblock);
     return 0;
   }
@@ -85,7 +85,8 @@ static int check(const int L[4], bool tblock) {
       }
     }
   }
-  printf("%dx%dx%dx%d tblock=%d: ok (%u blocks)\n", L[0], L[1], L[2], L[3], g.tblock, nb);
+  printf("%dx%dx%dx%d tblock=%d xwalk=%d: ok (%u blocks)\n", L[0], L[1], L[2], L[3], g.tblock,
+         g.xwalk, nb);
   return 0;
 }
 
@@ -97,7 +98,8 @@ int main() {
                           {64, 16, 16, 16}};
   int fails = 0;
   for (const auto &L : geoms)
-    for (int tb = 0; tb < 2; ++tb) fails += check(L, tb != 0);
+    for (int tb = 0; tb < 2; ++tb)
+      for (int xw = 0; xw < 2; ++xw) fails += check(L, tb != 0, xw != 0);
   // the multiply-high division against the exact quotient over its stated range
   for (uint32_t d = 1; d <= 2048; ++d) {
     const uint32_t m = fast_div_magic(d);

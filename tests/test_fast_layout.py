@@ -22,5 +22,7 @@ def test_fast_step_layout(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip().endswith("ALL OK")
-    assert "32x32x32x32 tblock=1: ok" in out.stdout
-    assert "8x8x8x8 tblock=1: ok" in out.stdout
+    assert "32x32x32x32 tblock=1 xwalk=0: ok" in out.stdout
+    assert "32x32x32x32 tblock=1 xwalk=1: ok" in out.stdout
+    assert "48x48x48x48 tblock=1 xwalk=1: ok" in out.stdout
+    assert "8x8x8x8 tblock=1 xwalk=0: ok" in out.stdout
