@@ -454,6 +454,15 @@ static int xwalk_pref() {
   if (v == -2) v = env_int("SU2GPU_XWALK", -1);
   return v;
 }
+// SU2GPU_NT = 1 / 0: non-temporal stores of the new links and momenta in the
+// specialised step; default: where the two link fields and the momenta exceed
+// the 256 MB Infinity Cache (48^4 FP32 466 against 496 us, 64^4 1331 against
+// 1446 us; at 32^4, which fits, 79 against 67 us)
+static int nt_pref() {
+  static int v = -2;
+  if (v == -2) v = env_int("SU2GPU_NT", -1);
+  return v;
+}
 // SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
 static int zs_pref() {
   static int v = -1;
@@ -510,6 +519,9 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     // +-z neighbours by lane exchange where every wave holds whole z-rows
     const bool zs = !site && zs_pref() && (64 % f.Lz == 0);
     const bool an = (L.c.flags & F_ANISO) != 0;
+    const int ntp = nt_pref();
+    const double fields = 4.0 * L.geo.V * (2.0 * sizeof(quat<T>) + 3.0 * sizeof(T));
+    const bool nt = ntp >= 0 ? ntp != 0 : fields > 256.0 * 1024 * 1024;
     dim3 grid((site ? 1 : 4) * (L.geo.V / NTF)), block(NTF);
 #define SU2_FAST(W, A)                                                                          \
   if (site)                                                                                     \
@@ -523,8 +535,16 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, false, true>), grid, block, 0, st,             \
                        (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
                        L.partials, L.pstride);                                                  \
+  else if (zs && nt)                                                                            \
+    hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, true, false, true>), grid, block, 0, st,       \
+                       (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
+                       L.partials, L.pstride);                                                  \
   else if (zs)                                                                                  \
     hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, true>), grid, block, 0, st,                    \
+                       (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
+                       L.partials, L.pstride);                                                  \
+  else if (nt)                                                                                  \
+    hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, false, false, true>), grid, block, 0, st,      \
                        (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c,            \
                        L.partials, L.pstride);                                                  \
   else                                                                                          \

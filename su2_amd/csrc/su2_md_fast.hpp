@@ -176,6 +176,26 @@ template <typename E> __device__ __forceinline__ E ldo(const E *__restrict__ bas
 template <typename E> __device__ __forceinline__ void sto(E *__restrict__ base, uint32_t i, E v) {
   *(E *)((char *)base + i * (uint32_t)sizeof(E)) = v;
 }
+// the same store with the non-temporal hint (streamed past the caches): for
+// lattices whose fields exceed the Infinity Cache
+typedef float su2_f4 __attribute__((ext_vector_type(4)));
+typedef double su2_d2 __attribute__((ext_vector_type(2)));
+template <typename E> __device__ __forceinline__ void sto_nt(E *__restrict__ base, uint32_t i, E v) {
+  __builtin_nontemporal_store(v, (E *)((char *)base + i * (uint32_t)sizeof(E)));
+}
+__device__ __forceinline__ void sto_nt(quat<float> *__restrict__ base, uint32_t i, quat<float> v) {
+  const su2_f4 w = {v.ar, v.ai, v.br, v.bi};
+  __builtin_nontemporal_store(w, (su2_f4 *)((char *)base + i * 16u));
+}
+__device__ __forceinline__ void sto_nt(quat<double> *__restrict__ base, uint32_t i, quat<double> v) {
+  const su2_d2 a = {v.ar, v.ai}, b = {v.br, v.bi};
+  __builtin_nontemporal_store(a, (su2_d2 *)((char *)base + i * 32u));
+  __builtin_nontemporal_store(b, (su2_d2 *)((char *)base + i * 32u + 16u));
+}
+template <bool NT, typename E> __device__ __forceinline__ void sto_h(E *__restrict__ base, uint32_t i, E v) {
+  if constexpr (NT) sto_nt(base, i, v);
+  else sto(base, i, v);
+}
 
 // u v as four FMA chains (16 FMA / MUL), conjugations of the operands passed in
 // through qdag and folded into the source negations of the FMAs
@@ -211,7 +231,7 @@ __device__ __forceinline__ void acc_dw(double (&K)[4], double w, const quat<T> &
 // Momentum and link update of one link from its staple sum K (FP64):
 // M = U K, P <- P - ep coef 2 Im(M) rounded once, U' = exp(eu P) U.
 // Returns Re(M) and the new momentum for the block sums.
-template <typename T, bool WRITE_U>
+template <typename T, bool WRITE_U, bool NT = false>
 __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (&K)[4], T pi0,
                                                 T pi1, T pi2, T *__restrict__ P0,
                                                 T *__restrict__ P1, T *__restrict__ P2,
@@ -225,9 +245,9 @@ __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (
   const T p0 = T(double(pi0) - f * Mbi);
   const T p1 = T(double(pi1) - f * Mbr);
   const T p2 = T(double(pi2) - f * Mai);
-  sto(P0, s, p0);
-  sto(P1, s, p1);
-  sto(P2, s, p2);
+  sto_h<NT>(P0, s, p0);
+  sto_h<NT>(P1, s, p1);
+  sto_h<NT>(P2, s, p2);
   pn[0] = p0;
   pn[1] = p1;
   pn[2] = p2;
@@ -260,7 +280,7 @@ __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (
       un = GSU2<T>::expmul(pn, T(c.eu), u);
       if (c.flags & F_RESTORE) un = GSU2<T>::restore(un);
     }
-    sto(Uo, s, un);
+    sto_h<NT>(Uo, s, un);
   }
   return ua * K[0] - ub * K[1] - uc * K[2] - ud * K[3];
 }
@@ -301,7 +321,8 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat
 // dimensions (direction ND - 1 the fastest); returns the input momentum (pi),
 // the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it reads
 // g.V and, with ZS, g.Lz (the extent of direction ND - 1).
-template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false, int ND = 4>
+template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false, int ND = 4,
+          bool NT = false>
 __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
@@ -388,11 +409,12 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
   pi[0] = pi0;
   pi[1] = pi1;
   pi[2] = pi2;
-  mar = md_su2_update<T, WRITE_U>(u, K, pi0, pi1, pi2, P0, P1, P2, Uout + (size_t)mu * V,
+  mar = md_su2_update<T, WRITE_U, NT>(u, K, pi0, pi1, pi2, P0, P1, P2, Uout + (size_t)mu * V,
                                   (uint32_t)s, c, pn);
 }
 
-template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false>
+template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false,
+          bool NT = false>
 __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
@@ -400,7 +422,7 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
                                             int pstride, double *lds) {
   T pi[3], pn[3];
   double mar;
-  md_su2_core<T, MU, WRITE_U, ZS, AN>(Uin, Uout, P, g, c, fs, pi, pn, mar);
+  md_su2_core<T, MU, WRITE_U, ZS, AN, 4, NT>(Uin, Uout, P, g, c, fs, pi, pn, mar);
   if (ACCUM) {
     double r0 = sq3(pi[0], pi[1], pi[2]), r1 = 2.0 * mar, r2 = sq3(pn[0], pn[1], pn[2]);
     block_sum3<NTF>(r0, r1, r2, lds);
@@ -416,7 +438,8 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
 // time (mu is uniform per block); 0 = one body with mu at run time
 // ZS (with SPEC; needs 64 % Lz == 0): +-z neighbours by lane exchange
 // AN: anisotropic weights w_t / w_s on the planes (FMA accumulation)
-template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false, bool AN = false>
+template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false, bool AN = false,
+          bool NT = false>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ Uin,
                                                  quat<T> *__restrict__ Uout,
                                                  T *__restrict__ P, FastGeo g, FusedCoef c,
@@ -425,13 +448,13 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ U
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
   if (SPEC) {
     switch (fs.mu) {
-      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
     }
   } else {
-    md_su2_body<T, -1, WRITE_U, ACCUM, false, AN>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
+    md_su2_body<T, -1, WRITE_U, ACCUM, false, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
   }
 }
 
