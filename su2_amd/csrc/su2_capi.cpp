@@ -214,11 +214,13 @@ su2k::FusedCoef base_coef(const L *h) {
 double *row(L *h, int r) { return h->partials + (size_t)r * h->row_stride; }
 
 // fused "P then U" (Uout != nullptr) or momentum-only step on raw buffers
-int fused_ptr(L *h, const void *Uin, void *Uout, void *P, su2k::FusedCoef c, int accum_row) {
+int fused_ptr(L *h, const void *Uin, void *Uout, void *P, su2k::FusedCoef c, int accum_row,
+              void *Pout = nullptr) {
   su2k::FusedLaunch fl;
   fl.Uin = Uin;
   fl.Uout = Uout;
   fl.P = P;
+  fl.Pout = Pout;
   fl.geo = h->geo;
   fl.c = c;
   fl.partials = nullptr;
@@ -1283,15 +1285,17 @@ void rk_coef(const L *h, double eps, double b[3], double c[3]) {
   }
 }
 
-// one RK3 stage on raw buffers: Z (+)= b D(Win); Wout = exp(c Z) Win (Wout may be null)
-int rk_stage(L *h, const void *Win, void *Wout, void *Z, double b, double c, bool first) {
+// one RK3 stage on raw buffers: Z (+)= b D(Win); Wout = exp(c Z) Win (Wout may be
+// null); with Zout the new Z goes there and Z is only read
+int rk_stage(L *h, const void *Win, void *Wout, void *Z, double b, double c, bool first,
+             void *Zout = nullptr) {
   su2k::FusedCoef k = base_coef(h);
   if (h->prec == 1 && h->group == 1) k.flags |= su2k::F_RESTORE;
   k.coef = b;
   k.ep = -1.0;
   k.eu = c;
   if (first) k.flags |= su2k::F_PZERO;
-  return fused_ptr(h, Win, Wout, Z, k, -1);
+  return fused_ptr(h, Win, Wout, Z, k, -1, Zout);
 }
 
 // full RK3 step Win -> Wout using tmp as the intermediate buffer
@@ -1435,15 +1439,12 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
       for (size_t k = k0; k < k1; ++k) {
         const size_t j = k - k0;
         // stage 0 ignores its Z input and writes Z0' in place; stages 1 and 2
-        // update a copy of the previous Z'
+        // read the previous Z' and write the new one to its own buffer
         CK(rk_stage(h, f.seg[j], f.stU[2 * j], f.stZ[3 * j], b[0], c[0], true));
-        HC(hipMemcpyAsync(f.stZ[3 * j + 1], f.stZ[3 * j], h->P_bytes, hipMemcpyDeviceToDevice,
-                          h->stream));
-        CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.stZ[3 * j + 1], b[1], c[1], false));
-        HC(hipMemcpyAsync(f.stZ[3 * j + 2], f.stZ[3 * j + 1], h->P_bytes,
-                          hipMemcpyDeviceToDevice, h->stream));
-        CK(rk_stage(h, f.stU[2 * j + 1], k + 1 < k1 ? f.seg[j + 1] : nullptr, f.stZ[3 * j + 2],
-                    b[2], c[2], false));
+        CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.stZ[3 * j], b[1], c[1], false,
+                    f.stZ[3 * j + 1]));
+        CK(rk_stage(h, f.stU[2 * j + 1], k + 1 < k1 ? f.seg[j + 1] : nullptr, f.stZ[3 * j + 1],
+                    b[2], c[2], false, f.stZ[3 * j + 2]));
       }
     } else {
       for (size_t k = k0; k + 1 < k1; ++k)
@@ -1463,11 +1464,9 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
       } else {
         // stage states: Z0' = b0 D(W0), W1 = exp(c0 Z0') W0, Z1' = Z0' + b1 D(W1),
         // W2 = exp(c1 Z1') W1, Z2' = Z1' + b2 D(W2)
-        CK(rk_stage(h, W0, f.W1, f.Z, b[0], c[0], true));
-        HC(hipMemcpyAsync(f.Zs[0], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
-        CK(rk_stage(h, f.W1, f.W2, f.Z, b[1], c[1], false));
-        HC(hipMemcpyAsync(f.Zs[1], f.Z, h->P_bytes, hipMemcpyDeviceToDevice, h->stream));
-        CK(rk_stage(h, f.W2, nullptr, f.Z, b[2], c[2], false));
+        CK(rk_stage(h, W0, f.W1, f.Zs[0], b[0], c[0], true));
+        CK(rk_stage(h, f.W1, f.W2, f.Zs[0], b[1], c[1], false, f.Zs[1]));
+        CK(rk_stage(h, f.W2, nullptr, f.Zs[1], b[2], c[2], false, f.Z));
       }
       for (int st = 2; st >= 0; --st) {
         {

@@ -562,10 +562,54 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
   }
 }
 
+// the step with a separate momentum output (FusedLaunch::Pout) in the
+// specialised kernel, where it applies (isotropic, no partial sums)
+template <class G, int ND>
+static bool launch_fast_pout(const FusedLaunch &L, hipStream_t st) {
+  using T = typename G::real;
+  if constexpr (ND != 4 || !std::is_same<G, GSU2<T>>::value) {
+    return false;
+  } else {
+    const int lvl = fast_level();
+    if (lvl == 0 || (sizeof(T) == 8 && lvl < 2)) return false;
+    if (L.c.flags & (F_PZERO | F_ROUND | F_ACCUM | F_ANISO)) return false;
+    if (fused_mode(L.geo) == 0 || fast_shape() == 1) return false;
+    FastGeo f;
+    const int xw = xwalk_pref();
+    const bool xwalk = xw >= 0 ? xw != 0 : (L.geo.L[2] * L.geo.L[3]) / NTF >= 16;
+    if (!make_fast_geo(L.geo.L, L.geo.V, tblock_pref() != 0, f, xwalk)) return false;
+    const bool zs = zs_pref() && (64 % f.Lz == 0);
+    const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
+#define SU2_POUT(W, Z)                                                                       \
+  hipLaunchKernelGGL((k_md_su2_4d_pout<T, W, Z>), grid, block, 0, st, (const quat<T> *)L.Uin, \
+                     (quat<T> *)L.Uout, (T *)L.P, (T *)L.Pout, f, L.c)
+    if (L.Uout) {
+      if (zs) SU2_POUT(true, true);
+      else SU2_POUT(true, false);
+    } else {
+      if (zs) SU2_POUT(false, true);
+      else SU2_POUT(false, false);
+    }
+#undef SU2_POUT
+    return true;
+  }
+}
+
 template <class G, int ND>
 static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
   using link = typename G::link;
   using T = typename G::real;
+  if (L.Pout) {
+    if (launch_fast_pout<G, ND>(L, st)) return;
+    // elsewhere: copy the momenta, then the in-place step on the copy
+    const size_t bytes = (size_t)L.geo.V * ND * G::NMOM * sizeof(T);
+    (void)hipMemcpyAsync(L.Pout, L.P, bytes, hipMemcpyDeviceToDevice, st);
+    FusedLaunch L2 = L;
+    L2.P = L.Pout;
+    L2.Pout = nullptr;
+    launch_fused_nd<G, ND>(L2, st);
+    return;
+  }
   if (launch_fast<G, ND>(L, st)) return;
   const int mode = fused_mode(L.geo);
   if (mode != 0) {

@@ -38,6 +38,7 @@ struct FusedLaunch {
   const void *Uin;
   void *Uout;  // nullptr: momentum-only step
   void *P;
+  void *Pout = nullptr;  // non-null: new momenta go here, P is only read
   Geo geo;
   FusedCoef c;
   double *partials;  // 3 rows of fused_blocks(geo) doubles (F_ACCUM) ...

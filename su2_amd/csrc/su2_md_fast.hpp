@@ -322,12 +322,12 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat
 // the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it reads
 // g.V and, with ZS, g.Lz (the extent of direction ND - 1).
 template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false, int ND = 4,
-          bool NT = false>
+          bool NT = false, bool POUT = false>
 __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
                                             const FastSite &fs, T (&pi)[3], T (&pn)[3],
-                                            double &mar) {
+                                            double &mar, T *__restrict__ Pw = nullptr) {
   const int mu = MU >= 0 ? MU : fs.mu;
   const int s = fs.s;
   const int *up = fs.up, *dn = fs.dn;
@@ -409,20 +409,25 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
   pi[0] = pi0;
   pi[1] = pi1;
   pi[2] = pi2;
-  mar = md_su2_update<T, WRITE_U, NT>(u, K, pi0, pi1, pi2, P0, P1, P2, Uout + (size_t)mu * V,
+  // POUT: the new momentum goes to Pw (same layout), P is only read
+  T *__restrict__ Q = POUT ? Pw : P;
+  mar = md_su2_update<T, WRITE_U, NT>(u, K, pi0, pi1, pi2, Q + (size_t)(mu * 3 + 0) * V,
+                                      Q + (size_t)(mu * 3 + 1) * V, Q + (size_t)(mu * 3 + 2) * V,
+                                      Uout + (size_t)mu * V,
                                   (uint32_t)s, c, pn);
 }
 
 template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false,
-          bool NT = false>
+          bool NT = false, bool POUT = false>
 __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
                                             quat<T> *__restrict__ Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
                                             const FastSite &fs, double *__restrict__ partials,
-                                            int pstride, double *lds) {
+                                            int pstride, double *lds,
+                                            T *__restrict__ Pw = nullptr) {
   T pi[3], pn[3];
   double mar;
-  md_su2_core<T, MU, WRITE_U, ZS, AN, 4, NT>(Uin, Uout, P, g, c, fs, pi, pn, mar);
+  md_su2_core<T, MU, WRITE_U, ZS, AN, 4, NT, POUT>(Uin, Uout, P, g, c, fs, pi, pn, mar, Pw);
   if (ACCUM) {
     double r0 = sq3(pi[0], pi[1], pi[2]), r1 = 2.0 * mar, r2 = sq3(pn[0], pn[1], pn[2]);
     block_sum3<NTF>(r0, r1, r2, lds);
@@ -455,6 +460,23 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ U
     }
   } else {
     md_su2_body<T, -1, WRITE_U, ACCUM, false, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
+  }
+}
+
+// The same step reading the momenta from P and writing the new ones to Pw (a
+// separate buffer of the same layout): the gradient-flow recomputation keeps
+// every stage's Z' without copying it first.
+template <typename T, bool WRITE_U, bool ZS>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_pout(const quat<T> *__restrict__ Uin,
+                                                      quat<T> *__restrict__ Uout,
+                                                      T *__restrict__ P, T *__restrict__ Pw,
+                                                      FastGeo g, FusedCoef c) {
+  const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
+  switch (fs.mu) {
+    case 0: md_su2_body<T, 0, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;
+    case 1: md_su2_body<T, 1, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;
+    case 2: md_su2_body<T, 2, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;
+    default: md_su2_body<T, 3, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;
   }
 }
 
