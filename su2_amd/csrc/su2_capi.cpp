@@ -105,6 +105,7 @@ struct su2gpu_lattice {
     void *lamW = nullptr, *lamZ = nullptr, *Gam = nullptr;
     void *R = nullptr;                  // scratch of the adjoint's plaquette pass
     std::vector<void *> stU, stZ;       // stage states of a segment: W1, W2 and Z0', Z1', Z2' per step
+    bool kept_last = false;             // the forward left the last segment's states in seg/stU/stZ
   } flow;
 };
 
@@ -1310,6 +1311,53 @@ int rk_step(L *h, const void *Win, void *Wout, void *tmp, void *Z, double eps) {
 
 }  // namespace
 
+namespace {
+// buffers for the stage states of one segment (seg: W_k, stU: W1 W2, stZ: the
+// three Z' per step), kept when they fit in half of the free device memory
+// and SU2GPU_FLOW_RECOMPUTE is not set; false: recompute per step instead
+int flow_keep_buffers(L *h, bool *keep) {
+  auto &f = h->flow;
+  const char *rc = std::getenv("SU2GPU_FLOW_RECOMPUTE");
+  *keep = !(rc && std::atoi(rc) != 0);
+  if (f.seg.size() < f.ckpt) {
+    size_t old = f.seg.size();
+    f.seg.resize(f.ckpt, nullptr);
+    for (size_t i = old; i < f.ckpt; ++i) HC(hipMalloc(&f.seg[i], h->U_bytes));
+  }
+  if (*keep && f.stU.size() < 2 * f.ckpt) {
+    size_t fr = 0, tot = 0;
+    HC(hipMemGetInfo(&fr, &tot));
+    const size_t need = (2 * f.ckpt - f.stU.size()) * h->U_bytes +
+                        (3 * f.ckpt - f.stZ.size()) * h->P_bytes;
+    *keep = need < fr / 2;
+    if (*keep) {
+      while (f.stU.size() < 2 * f.ckpt) {
+        f.stU.push_back(nullptr);
+        HC(hipMalloc(&f.stU.back(), h->U_bytes));
+      }
+      while (f.stZ.size() < 3 * f.ckpt) {
+        f.stZ.push_back(nullptr);
+        HC(hipMalloc(&f.stZ.back(), h->P_bytes));
+      }
+    }
+  }
+  return SU2GPU_OK;
+}
+
+// step k0 + j of a segment stage by stage from seg[j], keeping W1, W2 and the
+// three Z'; the new W goes to Wout
+int flow_kept_step(L *h, size_t j, void *Wout, const double b[3], const double c[3]) {
+  auto &f = h->flow;
+  // stage 0 ignores its Z input and writes Z0' in place; stages 1 and 2 read
+  // the previous Z' and write the new one to its own buffer
+  CK(rk_stage(h, f.seg[j], f.stU[2 * j], f.stZ[3 * j], b[0], c[0], true));
+  CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.stZ[3 * j], b[1], c[1], false,
+              f.stZ[3 * j + 1]));
+  CK(rk_stage(h, f.stU[2 * j + 1], Wout, f.stZ[3 * j + 1], b[2], c[2], false, f.stZ[3 * j + 2]));
+  return SU2GPU_OK;
+}
+}  // namespace
+
 int su2gpu_flow_forward(su2gpu_lattice *h, double eps, size_t nsteps, size_t ckpt_every,
                         int observable, double *value) {
   CK(check(h));
@@ -1329,17 +1377,39 @@ int su2gpu_flow_forward(su2gpu_lattice *h, double eps, size_t nsteps, size_t ckp
   CK(flow_alloc(h, &f.W1, h->U_bytes));
   CK(flow_alloc(h, &f.Z, h->P_bytes));
   HC(hipMemcpyAsync(f.W[0], h->U[h->cur], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+  f.valid = false;
+  f.kept_last = false;
+  f.eps = eps;
+  f.nsteps = nsteps;
+  f.ckpt = ckpt_every;
+  // the last segment, the first one the backward pass needs, runs stage by
+  // stage into the kept-state buffers, so the backward does not recompute it
+  bool keep = false;
+  if (nsteps > 0) CK(flow_keep_buffers(h, &keep));
+  const size_t klast = nck > 0 ? (nck - 1) * ckpt_every : 0;
+  double rb[3], rc3[3];
+  rk_coef(h, eps, rb, rc3);
   int a = 0;
   for (size_t k = 0; k < nsteps; ++k) {
     if (k % ckpt_every == 0)
       HC(hipMemcpyAsync(f.ckpts[k / ckpt_every], f.W[a], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
-    CK(rk_step(h, f.W[a], f.W[1 - a], f.W1, f.Z, eps));
-    a = 1 - a;
+    if (keep && k >= klast) {
+      const size_t j = k - klast;
+      if (j == 0)
+        HC(hipMemcpyAsync(f.seg[0], f.W[a], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+      if (k + 1 < nsteps) {
+        CK(flow_kept_step(h, j, f.seg[j + 1], rb, rc3));
+      } else {
+        CK(flow_kept_step(h, j, f.W[1 - a], rb, rc3));
+        a = 1 - a;
+      }
+    } else {
+      CK(rk_step(h, f.W[a], f.W[1 - a], f.W1, f.Z, eps));
+      a = 1 - a;
+    }
   }
+  f.kept_last = keep;
   f.fin = a;
-  f.eps = eps;
-  f.nsteps = nsteps;
-  f.ckpt = ckpt_every;
   f.obs = observable;
   // observable at W(t): t^2 E_clover(t) or the Wilson action S (get_S_G_hmc)
   const int save = h->cur;
@@ -1376,11 +1446,6 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
   CK(flow_alloc(h, &f.lamW, h->P_bytes));
   CK(flow_alloc(h, &f.lamZ, h->P_bytes));
   CK(flow_alloc(h, &f.R, (size_t)h->V * 36 * h->real_bytes));
-  if (f.seg.size() < f.ckpt) {
-    size_t old = f.seg.size();
-    f.seg.resize(f.ckpt, nullptr);
-    for (size_t i = old; i < f.ckpt; ++i) HC(hipMalloc(&f.seg[i], h->U_bytes));
-  }
   const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
   // cotangent of W(t)
   const double t = f.eps * (double)f.nsteps;
@@ -1404,26 +1469,10 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
   // kept from its forward recomputation when they fit in half of the free
   // device memory; otherwise every step recomputes its stages from W_k (the
   // same kernels on the same inputs: identical results, one more forward pass).
-  // SU2GPU_FLOW_RECOMPUTE=1 forces the recomputation.
-  const char *rc = std::getenv("SU2GPU_FLOW_RECOMPUTE");
-  bool keep = !(rc && std::atoi(rc) != 0);
-  if (keep && f.stU.size() < 2 * f.ckpt) {
-    size_t fr = 0, tot = 0;
-    HC(hipMemGetInfo(&fr, &tot));
-    const size_t need = (2 * f.ckpt - f.stU.size()) * h->U_bytes +
-                        (3 * f.ckpt - f.stZ.size()) * h->P_bytes;
-    keep = need < fr / 2;
-    if (keep) {
-      while (f.stU.size() < 2 * f.ckpt) {
-        f.stU.push_back(nullptr);
-        HC(hipMalloc(&f.stU.back(), h->U_bytes));
-      }
-      while (f.stZ.size() < 3 * f.ckpt) {
-        f.stZ.push_back(nullptr);
-        HC(hipMalloc(&f.stZ.back(), h->P_bytes));
-      }
-    }
-  }
+  // SU2GPU_FLOW_RECOMPUTE=1 forces the recomputation.  The forward pass left
+  // the last segment's states in place (kept_last).
+  bool keep = false;
+  CK(flow_keep_buffers(h, &keep));
   // The gather of a stage's Hessian-vector product (the b/c role terms in R)
   // runs fused with the pointwise adjoint of the stage before it; the stage
   // recomputation in between touches neither R nor lamW / lamZ.
@@ -1433,18 +1482,16 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
   for (size_t sg = nseg; sg-- > 0;) {
     const size_t k0 = sg * f.ckpt, k1 = std::min(f.nsteps, k0 + f.ckpt);
     // recompute W_k, k0 <= k < k1, from the checkpoint
-    HC(hipMemcpyAsync(f.seg[0], f.ckpts[sg], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
-    if (keep) {
+    const bool have = keep && f.kept_last && sg + 1 == nseg;
+    if (!have)
+      HC(hipMemcpyAsync(f.seg[0], f.ckpts[sg], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+    if (have) {
+      // the forward pass left this segment's states in place
+    } else if (keep) {
       // stage by stage, keeping W1, W2 and the three Z'
       for (size_t k = k0; k < k1; ++k) {
         const size_t j = k - k0;
-        // stage 0 ignores its Z input and writes Z0' in place; stages 1 and 2
-        // read the previous Z' and write the new one to its own buffer
-        CK(rk_stage(h, f.seg[j], f.stU[2 * j], f.stZ[3 * j], b[0], c[0], true));
-        CK(rk_stage(h, f.stU[2 * j], f.stU[2 * j + 1], f.stZ[3 * j], b[1], c[1], false,
-                    f.stZ[3 * j + 1]));
-        CK(rk_stage(h, f.stU[2 * j + 1], k + 1 < k1 ? f.seg[j + 1] : nullptr, f.stZ[3 * j + 1],
-                    b[2], c[2], false, f.stZ[3 * j + 2]));
+        CK(flow_kept_step(h, j, k + 1 < k1 ? f.seg[j + 1] : nullptr, b, c));
       }
     } else {
       for (size_t k = k0; k + 1 < k1; ++k)
