@@ -416,6 +416,7 @@ def test_fp32_specialised_step_32(S, oracle):
     (8, 4, 24, 12, 4),   # neither
     (2, 4, 32, 8, 4),    # Lx = 2, Ly = 4: x and y neighbours wrap within a block
     (16, 16, 1, 8, 4),   # Lz = 1: the z neighbours are the link itself (lane exchange with itself)
+    (2, 64, 64, 8, 4),   # an x plane of 16 blocks: the x-fastest walk of the 3-volume blocks
 ])
 def test_fp32_specialised_step_geometries(S, oracle, dims):
     """The specialised FP32 step on the launch layouts other than 32^4's (every
