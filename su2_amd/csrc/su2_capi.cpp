@@ -1327,8 +1327,17 @@ int flow_keep_buffers(L *h, bool *keep) {
   if (*keep && f.stU.size() < 2 * f.ckpt) {
     size_t fr = 0, tot = 0;
     HC(hipMemGetInfo(&fr, &tot));
+    // the backward pass's own fields (W2, Zs[2], lamW, lamZ, the 36 V scratch
+    // R and the 18 V clover cotangent Gam) count against the same free memory
+    // when it has not allocated them yet
+    size_t bwd = 0;
+    if (!f.W2) bwd += h->U_bytes;
+    for (void *p : {f.Zs[0], f.Zs[1], f.lamW, f.lamZ})
+      if (!p) bwd += h->P_bytes;
+    if (!f.R) bwd += (size_t)h->V * 36 * h->real_bytes;
+    if (!f.Gam) bwd += (size_t)h->V * 18 * h->real_bytes;
     const size_t need = (2 * f.ckpt - f.stU.size()) * h->U_bytes +
-                        (3 * f.ckpt - f.stZ.size()) * h->P_bytes;
+                        (3 * f.ckpt - f.stZ.size()) * h->P_bytes + bwd;
     *keep = need < fr / 2;
     if (*keep) {
       while (f.stU.size() < 2 * f.ckpt) {
@@ -1541,6 +1550,10 @@ int su2gpu_flow_backward(su2gpu_lattice *h, double *grad) {
     su2k::launch_flow_gather(h->prec, f.R, f.lamW, h->geo, pending_b, h->stream);
     CK(last_launch("flow_gather"));
   }
+  // With more than one segment the buffers now hold the first segment's
+  // states, not the last one's: a further backward of the same forward must
+  // recompute the last segment.
+  if (nseg > 1) f.kept_last = false;
   return get_mom_host(h, f.lamW, grad);
 }
 

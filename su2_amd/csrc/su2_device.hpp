@@ -105,8 +105,11 @@ __device__ __forceinline__ void cos_sinc(double x, double &c, double &s) {
     s = 1.0 + x * (-1.0 / 6 + x * (1.0 / 120 + x * (-1.0 / 5040 + x * (1.0 / 362880))));
     return;
   }
+  // at most 520 halvings: every finite x <= 1.8e308 is below 2.5 by then, and
+  // an infinite angle (a diverged momentum) ends the loop with a NaN result
+  // instead of spinning
   int k = 0;
-  while (x > 2.5) {
+  while (x > 2.5 && k < 520) {
     x *= 0.25;
     ++k;
   }
