@@ -8,11 +8,13 @@ refresh, H_old, 20 OMF2 steps (41 force evaluations, 40 link updates), final
 restoreSU, H_new, accept/reject.  One trajectory performs 4 V * 20
 link-updates.
 
-`value` is the FP32 form's link-updates/s (links, staples, forces and momenta in
-FP32, compensated staple sums, reunitarisation every step, FP64 action /
-kinetic sums and accept/reject).  The plain FP64 form is timed the same way and
-reported under "fp64", with the FP32-vs-FP64 dH differences from identical
-starts and momenta.
+`value` is the plain FP64 form's link-updates/s (the reference's precision).
+The FP32 form (links, staples, forces and momenta in FP32, FP64 staple sums,
+reunitarisation every step, FP64 action / kinetic sums and accept/reject) is
+timed the same way and reported under "fp32", with its dH differences against
+the FP64 form from identical starts and momenta at 32^4 and whether they meet
+|delta dH| <= 1e-6.  The 48^4 leg ("cfg48") times both forms beyond the 256 MB
+Infinity Cache and repeats the dH comparison at beta = 2.5.
 
 Prints ONE JSON line.
 """
@@ -252,21 +254,32 @@ def metropolis(S, L=32, beta=2.4, n_hit=10, delta=0.3, sweeps=4, seed=4243):
     return out
 
 
-def cpu_baseline(L, nsteps, beta, seed):
-    """The reference's own md_update (oracle/_ref build of its hot-path headers),
-    one OMF2 trajectory at the benchmark size on all host cores; falls back to
-    the C oracle when the reference build is absent."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def affinity_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_sample(L, nsteps, beta, seed):
+    """One OMF2(nsteps) trajectory of the reference's md_update (oracle/_ref
+    build of its hot-path headers; the C oracle where that build is absent) on
+    an L^4 Haar start, with the OpenMP thread count of this process's
+    environment.  Runs in a child process (cpu_baseline) and prints one JSON
+    line."""
     from oracle import oracle as orc
     dims = (L, L, L, L, 4)
-    ncores = os.cpu_count() or 1
-    try:
-        import psutil
-        ncores = len(psutil.Process().cpu_affinity())
-    except Exception:
-        pass
-    ncores = min(ncores, 16)
-    os.environ["OMP_NUM_THREADS"] = str(ncores)
     kind = "reference" if orc.have_reference() else "port"
     lib = orc.Reference() if kind == "reference" else orc.Oracle()
     U = orc.Oracle().haarstart(orc.SU2, dims, seed)
@@ -284,10 +297,42 @@ def cpu_baseline(L, nsteps, beta, seed):
         os.dup2(saved, 1)
         os.close(saved)
         os.close(devnull)
-    return {"value": 4 * (L ** 4) * nsteps / dt, "unit": "link-updates/s", "cores": ncores,
-            "kind": kind,
-            "sample": f"1 OMF2({nsteps}) trajectory of md_update at {L}^4 (incl. momenta, H, copy), "
-                      f"{dt:.1f} s wall"}
+    print(json.dumps({"kind": kind, "seconds": dt, "link_updates_per_s": 4 * L ** 4 * nsteps / dt}),
+          flush=True)
+
+
+def cpu_baseline(L, nsteps, beta, seed):
+    """The reference CPU path timed on this host: the reference's own md_update
+    (its hot-path headers compiled by oracle/Makefile into oracle/_ref) on
+    every core of this process's affinity mask for one trajectory at the
+    benchmark size, and on one thread for one trajectory at (L/2)^4 (the same
+    per-link work, 1/16 of the sites, so that the sample stays ~10 s).  Each
+    sample runs in a child process with its own OMP_NUM_THREADS.  The
+    reference's force loop is an OpenMP loop over t (flat-gaugemonomial.hh:
+    109-110), i.e. L work items."""
+    import subprocess
+    ncores = affinity_cores()
+
+    def sample(Ls, threads):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", str(Ls),
+                              "--nsteps", str(nsteps), "--beta", str(beta), "--seed", str(seed)],
+                             env=env, capture_output=True, text=True, timeout=600)
+        if out.returncode != 0:
+            raise RuntimeError("cpu sample failed: " + out.stderr[-2000:])
+        return json.loads(out.stdout.strip().splitlines()[-1])
+
+    full = sample(L, ncores)
+    one = sample(L // 2, 1)
+    return {"value": full["link_updates_per_s"], "unit": "link-updates/s", "cores": ncores,
+            "kind": full["kind"],
+            "sample": f"1 OMF2({nsteps}) trajectory of md_update at {L}^4 on {ncores} threads "
+                      f"(incl. momenta, H, copy), {full['seconds']:.1f} s wall",
+            "value_1thread": one["link_updates_per_s"],
+            "sample_1thread": f"1 OMF2({nsteps}) trajectory at {L // 2}^4 on 1 thread, "
+                              f"{one['seconds']:.1f} s wall",
+            "cpu_model": cpu_model(),
+            "note": "the reference parallelises its force sweep over t only (L work items)"}
 
 
 def measured_traffic(L, form):
@@ -305,6 +350,25 @@ def measured_traffic(L, form):
     return None if rec is None else rec.get("hbm_bytes_per_launch")
 
 
+def kernel_name(prec, L):
+    """Instantiation of the specialised step that a P-then-U launch of the
+    L^4 SU(2) lattice takes (su2_kernels.hip launch_fast): template flags
+    <T, WRITE_U, ACCUM, SPEC, ZS (64 % L == 0), AN, NT (fields > 256 MB)>."""
+    t = "double" if prec == 0 else "float"
+    r = 8 if prec == 0 else 4
+    zs = "true" if 64 % L == 0 else "false"
+    nt = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
+    return f"k_md_su2_4d<{t},true,false,1,{zs},false" + (",true>" if nt else ">")
+
+
+def form_roofline(f, prec, L):
+    return {"bound": "hbm", "achieved": f["fused_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": f["fused_achieved_GBs"] / HBM_PEAK_GBS, "kernel": kernel_name(prec, L),
+            "avg_us": f["fused_avg_us"], "bytes_per_launch": f["fused_bytes_per_launch"],
+            "compulsory_frac": f["fused_compulsory_GBs"] / HBM_PEAK_GBS,
+            "trajectory_frac": f["traj_achieved_GBs"] / HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,16 +381,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--ddh-traj", type=int, default=8, help="paired FP32/FP64 trajectories")
     ap.add_argument("--chain", type=int, default=100, help="chain length of the acceptance comparison")
-    ap.add_argument("--prec-L", type=int, default=48, help="lattice extent of the FP32-vs-FP64 study")
-    ap.add_argument("--prec-beta", type=float, default=2.5, help="beta of the FP32-vs-FP64 study")
+    ap.add_argument("--prec-L", type=int, default=48, help="lattice extent of the 48^4 leg")
+    ap.add_argument("--prec-beta", type=float, default=2.5, help="beta of the 48^4 leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-flow", action="store_true", help="skip the flow-adjoint leg")
-    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 study")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32-vs-FP64 studies")
+    ap.add_argument("--no-48", action="store_true", help="skip the 48^4 leg")
     ap.add_argument("--no-metropolis", action="store_true", help="skip the Metropolis-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--scan", type=str, default="",
                     help="comma-separated extents L: also time both forms on L^4 (scan entry)")
+    ap.add_argument("--cpu-sample", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_sample:
+        cpu_sample(args.cpu_sample, args.nsteps, args.beta, args.seed)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
               "the lattice is not decomposed across GPUs", file=sys.stderr)
@@ -334,10 +403,24 @@ def main():
 
     import su2_amd as S
     S.load_library()
-    f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
     f64 = run_form(S, S.FP64, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
-    ddh = None if args.no_precision else delta_dH(S, args.prec_L, args.nsteps, args.prec_beta,
-                                                   args.seed, args.ddh_traj, nchain=args.chain)
+    f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
+    ddh32 = ddh48 = None
+    if not args.no_precision:
+        ddh32 = delta_dH(S, args.L, args.nsteps, args.beta, args.seed, args.ddh_traj,
+                         nchain=args.chain)
+    leg48 = None
+    if not args.no_48:
+        Lp = args.prec_L
+        leg48 = {"lattice": [Lp] * 4, "beta": args.prec_beta}
+        for name, prec in (("fp64", S.FP64), ("fp32", S.FP32)):
+            r = run_form(S, prec, Lp, args.nsteps, 3, 1, args.seed, args.prec_beta)
+            leg48[name] = dict(r, roofline=form_roofline(r, prec, Lp),
+                               traffic=measured_traffic(Lp, name))
+        if not args.no_precision:
+            ddh48 = delta_dH(S, Lp, args.nsteps, args.prec_beta, args.seed, args.ddh_traj,
+                             nchain=args.chain)
+            leg48["delta_dH"] = ddh48
     copy_ms = S.copy_probe(1 << 30, 20)
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
@@ -351,45 +434,48 @@ def main():
             r = run_form(S, prec, L, args.nsteps, 3, 1, args.seed, args.beta)
             e[name] = {"link_updates_per_s": r["link_updates_per_s"], "ms_per_traj": r["ms_per_traj"],
                        "fused_avg_us": r["fused_avg_us"],
+                       "fused_roofline_frac": r["fused_achieved_GBs"] / HBM_PEAK_GBS,
                        "fused_link_updates_per_s": 4 * L ** 4 / (r["fused_avg_us"] * 1e-6)}
         scan.append(e)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
     n = args.nsteps
-    kname = "k_md_su2_4d<float,true,false,1,true,false>"
+    roof = form_roofline(f64, 0, args.L)
+    roof["traffic"] = measured_traffic(args.L, "fp64")
+    roof["traffic_source"] = "profiles/fused_step_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE)"
+    fp32 = dict(f32, roofline=form_roofline(f32, 1, args.L), traffic=measured_traffic(args.L, "fp32"))
+    if ddh32 is not None:
+        worst = max(ddh32["max_abs_dH32_minus_dH64"], ddh32["max_abs_ddH_fp32"])
+        fp32["delta_dH"] = ddh32
+        fp32["delta_dH_max"] = worst
+        fp32["delta_dH_target"] = 1e-6
+        fp32["delta_dH_target_met"] = bool(worst <= 1e-6)
     out = {
         "metric": "SU(2) 32^4 HMC link-updates/s, %HBM roofline; FP32 vs FP64 form at |dDH|<=1e-6",
-        "value": f32["link_updates_per_s"],
+        "value": f64["link_updates_per_s"],
         "unit": "link-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": f32["ms_per_traj"],
+        "ms_per_step": f64["ms_per_traj"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"SU(2) d=4 {args.L}^4 periodic, beta={args.beta}, OMF2 {n} steps, "
-                               f"tau=1, Haar hot start, device Gaussian momenta, FP32 form",
+                               f"tau=1, Haar hot start, device Gaussian momenta, plain FP64 form "
+                               f"(the reference's precision)",
                    "lattice": [args.L] * 4, "integrator": "omf2", "n_steps": n, "tau": 1.0},
-        "roofline": {"bound": "hbm", "achieved": f32["fused_achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": f32["fused_achieved_GBs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.L, "fp32"),
-                     "kernel": kname,
-                     "avg_us": f32["fused_avg_us"], "bytes_per_launch": f32["fused_bytes_per_launch"],
-                     "compulsory_frac": f32["fused_compulsory_GBs"] / HBM_PEAK_GBS,
-                     "trajectory_frac": f32["traj_achieved_GBs"] / HBM_PEAK_GBS},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "fp32": f32,
-        "fp64": dict(f64, fused_roofline_frac=f64["fused_achieved_GBs"] / HBM_PEAK_GBS,
-                     trajectory_frac=f64["traj_achieved_GBs"] / HBM_PEAK_GBS,
-                     traffic=measured_traffic(args.L, "fp64")),
+        "fp64": f64,
+        "fp32": fp32,
         "fp32_over_fp64": f32["link_updates_per_s"] / f64["link_updates_per_s"],
-        "delta_dH": ddh,
+        "cfg48": leg48,
         "measured_copy_GBs": copy_gbs,
-        "fused_compulsory_frac_of_measured_copy_fp32": f32["fused_compulsory_GBs"] / copy_gbs,
+        "fused_compulsory_frac_of_measured_copy_fp64": f64["fused_compulsory_GBs"] / copy_gbs,
         "batch": batch,
         "flow_adjoint": flow,
         "metropolis": metro,

@@ -326,7 +326,10 @@ ORC_API void orc_random_gauge_trafo(int grp, const int *dims, double *U, uint64_
 
 /* Haar-uniform SU(2) start (not in the reference): each link is a normalised
  * 4-vector of normal_distribution draws (Re a, Im a, Re b, Im b) from
- * mt19937(seed), links in storage order.  U(1): uniform phase in [-pi,pi). */
+ * mt19937(seed), links in storage order.  U(1): uniform phase in [-pi,pi).
+ * Pinned by the reference build's normal stream (the first links are its
+ * normalised 4-tuples) and by the Haar measure's moments, component law and
+ * left invariance (tests/test_oracle.py). */
 ORC_API void orc_haarstart(int grp, const int *dims, double *U, uint64_t seed) {
   geom g = mkgeom(dims);
   orc_mt19937 e;

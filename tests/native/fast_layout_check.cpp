@@ -100,6 +100,37 @@ int main() {
   for (const auto &L : geoms)
     for (int tb = 0; tb < 2; ++tb)
       for (int xw = 0; xw < 2; ++xw) fails += check(L, tb != 0, xw != 0);
+  // Regression: a divisor of 1 has no 32-bit multiply-high constant
+  // (floor(2^32 / 1) + 1 = 2^32 + 1 truncates to 1, and umulhi(n, 1) = 0).  With
+  // Lt = 8 the t-chunk per XCD is Tc = 1, so that constant would give q = 0 for
+  // r = 1 and t = xcd + 1 = 8 = Lt for the last XCD: loads past the link field
+  // and momentum / link stores up to 512 elements past the end of the mu = 3
+  // planes (the 8^4 flow-stage launches of the specialised step).  fast_div
+  // codes d = 1 as m = 0; the truncated constant must still be caught here.
+  {
+    const int L8[4] = {8, 8, 8, 8};
+    FastGeo g;
+    if (!make_fast_geo(L8, 4096, true, g) || g.Tc != 1 || g.m_Tc != 0) {
+      printf("FAIL 8^4 layout: Tc = %d, m_Tc = %u\n", g.Tc, g.m_Tc);
+      return 1;
+    }
+    g.m_Tc = (uint32_t)((((uint64_t)1) << 32) / 1 + 1);  // the truncated constant: 1
+    int max_s = -1;
+    for (uint32_t b = 0; b < 64; ++b)
+      for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
+        const FastSite f = fast_site(g, b, 64, tid);
+        if (f.s > max_s) max_s = f.s;
+      }
+    if (max_s < 4096) {
+      printf("FAIL the truncated d = 1 constant stays in range (max s %d)\n", max_s);
+      return 1;
+    }
+    printf("8x8x8x8 truncated d=1 multiplier: site %d >= V = 4096 (caught)\n", max_s);
+    if (fast_div(12345u, fast_div_magic(1)) != 12345u) {
+      printf("FAIL fast_div by 1\n");
+      return 1;
+    }
+  }
   // the multiply-high division against the exact quotient over its stated range
   for (uint32_t d = 1; d <= 2048; ++d) {
     const uint32_t m = fast_div_magic(d);

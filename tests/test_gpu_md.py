@@ -27,6 +27,16 @@ def S():
     return su2_amd
 
 
+def fp32_dH_bound(dims):
+    """Bound on |dH_FP32 - dH_FP64| for one trajectory: rounding the N stored
+    FP32 degrees of freedom (4 per link, 3 per momentum) perturbs H by about
+    sqrt(N) * 2^-24 * |dH/dx| per trajectory with |dH/dx| = O(1); the bound
+    takes 20 times sqrt(N) 2^-24 (6.5e-3 at 32^4, 1.5e-2 at 48^4), a tenth of
+    the earlier fixed 0.05 at 32^4."""
+    V = dims[0] * dims[1] * dims[2] * dims[3]
+    return 20.0 * math.sqrt(V * dims[4] * 7) * 2.0 ** -24
+
+
 def _start(oracle, grp, dims, seed=SEED, delta=1.0):
     U = oracle.hotstart(grp, dims, seed, delta)
     return oracle.random_gauge_trafo(grp, dims, U, 654321)
@@ -405,8 +415,11 @@ def test_fp32_specialised_step_32(S, oracle):
     assert r32["gauge_old"] == pytest.approx(s_start, rel=1e-12)
     assert r32["gauge_new"] == pytest.approx(s_end, rel=1e-12)
     assert r32["kin_old"] == pytest.approx(r64["kin_old"], rel=1e-7)
-    assert np.abs(U32 - U64).max() < 1e-4
-    assert abs(r32["deltaH"] - r64["deltaH"]) < 0.05
+    dU = np.abs(U32 - U64).max()
+    ddh = abs(r32["deltaH"] - r64["deltaH"])
+    print(f"32^4 FP32 vs FP64 OMF2(10): max|dU| = {dU:.3e}, |dH32 - dH64| = {ddh:.3e}")
+    assert dU < 1e-4
+    assert ddh < fp32_dH_bound(dims)
 
 
 # dims = (Lx, Ly, Lz, Lt, nd)
@@ -614,3 +627,86 @@ def test_large_lattice_64(S, prec):
         assert lat.gauge_action() == pytest.approx(s1, rel=1e-9 if prec == 0 else 1e-5)
         q = lat.get_links().reshape(-1, 4)
         assert np.abs((q ** 2).sum(1) - 1.0).max() < (1e-14 if prec == 0 else 5e-7)
+
+
+def test_fp64_and_fp32_trajectory_parity_48(S, oracle):
+    """SU(2) 48^4, beta = 2.5: one OMF2(20) trajectory from the seeded Haar hot
+    start with the reference's host momentum stream (initnormal, mt19937), in
+    both forms against the oracle's FP64 trajectory (md_update.hh:18-81,
+    integrator.hh:83-115).  The FP64 form: <= 1e-10 per link, dH and the
+    plaquette to 1e-9 relative.  The FP32 form (at 48^4 the step runs with
+    non-temporal stores and without the z lane exchange, 64 % 48 != 0): <= 1e-4
+    per link, dH within the sqrt(N) rounding bound of the FP32 fields, the
+    plaquette sum within 1e-8 relative."""
+    dims = (48, 48, 48, 48, 4)
+    beta = 2.5
+    U = oracle.haarstart(1, dims, SEED)
+    P = oracle.initnormal(1, dims, SEED + 1)
+    Uw, Pw = oracle.integrate(1, dims, U, P, beta, 6, 20)
+    h_old = oracle.action(1, dims, U, beta, compensated=True) + oracle.kinetic(1, dims, P, compensated=True)
+    h_new = oracle.action(1, dims, Uw, beta, compensated=True) + oracle.kinetic(1, dims, Pw, compensated=True)
+    dh_w = h_new - h_old
+    plaq_w = oracle.retr_sum_plaq(1, dims, Uw, compensated=True)
+    res = {}
+    for prec in (S.FP64, S.FP32):
+        with S.Lattice(S.SU2, dims, beta, precision=prec) as lat:
+            lat.set_links(U)
+            r = lat.md_trajectory(P, 6, 20)
+            lat.md_finish(True)
+            Ug = lat.get_links()
+            plaq = lat.gauge_energy()
+        res[prec] = (np.abs(Ug - Uw).max(), r["deltaH"], plaq)
+        del Ug
+    d64, dh64, pl64 = res[S.FP64]
+    d32, dh32, pl32 = res[S.FP32]
+    print(f"48^4 OMF2(20): dH oracle {dh_w:.10f}; FP64 max|dU| {d64:.3e}, dH {dh64:.10f}; "
+          f"FP32 max|dU| {d32:.3e}, dH {dh32:.6f} (|diff| {abs(dh32 - dh_w):.3e}), "
+          f"plaquette rel diff {abs(pl32 - plaq_w) / abs(plaq_w):.3e}")
+    assert d64 <= 1e-10
+    assert dh64 == pytest.approx(dh_w, rel=1e-9, abs=1e-8)
+    assert pl64 == pytest.approx(plaq_w, rel=1e-9)
+    assert d32 <= 1e-4
+    assert abs(dh32 - dh_w) < fp32_dH_bound(dims)
+    assert pl32 == pytest.approx(plaq_w, rel=1e-8)
+
+
+_NT_CHILD = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import su2_amd as S
+from oracle.oracle import Oracle
+orc = Oracle()
+for dims in [(8, 8, 8, 8, 4), (2, 64, 64, 8, 4)]:
+    U = orc.hotstart(1, dims, 77, 0.3)
+    P = orc.initnormal(1, dims, 78)
+    Uw, _ = orc.integrate(1, dims, U, P, 2.4, 6, 5)
+    for prec in (S.FP64, S.FP32):
+        with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
+            lat.set_links(U)
+            lat.md_trajectory(P, 6, 5)
+            lat.md_finish(True)
+            d = float(np.abs(lat.get_links() - Uw).max())
+        tol = 1e-11 if prec == S.FP64 else 2e-5
+        print(dims, prec, d, flush=True)
+        assert d < tol, (dims, prec, d)
+print("NT_CHILD_OK", flush=True)
+"""
+
+
+@pytest.mark.parametrize("env", [{"SU2GPU_NT": "1", "SU2GPU_XWALK": "1"},
+                                 {"SU2GPU_NT": "1", "SU2GPU_XWALK": "0"}],
+                         ids=["nt_xwalk", "nt"])
+def test_nontemporal_store_step_parity(env):
+    """The specialised step with its non-temporal stores forced on (quat<float>
+    and quat<double> sto_nt) on small lattices, in a child process because the
+    switches are read once per process: both forms against the oracle's FP64
+    trajectory from the same links and host momenta."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    e = dict(os.environ, **env)
+    out = subprocess.run([sys.executable, "-c", _NT_CHILD, root], env=e, capture_output=True,
+                         text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "NT_CHILD_OK" in out.stdout

@@ -375,3 +375,99 @@ def test_two_dimensional_plaquette_exact_metropolis(oracle, grp, beta, nu, delta
     err = blocks.std(ddof=1) / math.sqrt(len(blocks))
     assert abs(plaq.mean() - exact) < 4 * err + 5e-4, (plaq.mean(), exact, err)
     assert 0.2 < rate[0] < 0.9
+
+
+# ------------------------------------------------------------- Haar start ---
+# orc_haarstart has no reference counterpart (the reference's hot start is
+# random_element, random_element.hh:9-25, whose q0 = cos(alpha) is not Haar
+# distributed).  It is pinned by (1) its draw order: link i is the i-th
+# 4-tuple of the reference's normal_distribution stream over mt19937(seed)
+# (the golden 1024 normals written from the reference build), normalised; and
+# (2) what the Haar measure of SU(2) = the uniform measure on S^3 fixes:
+# E[q_k^2] = 1/4 and E[q_k^4] = 1/8 for every component, the density
+# (2/pi) sqrt(1 - x^2) of each component (Kolmogorov-Smirnov), invariance of
+# that law under left multiplication by a fixed group element, and
+# independence of neighbouring links.  A start with the reference's
+# random_element recipe (E[q0^2] = 1/2), with uniform components normalised
+# (E[q^4] = 0.139 instead of 0.125), or with three normals and a derived fourth
+# component fails one of these.
+
+def _semicircle_cdf(x):
+    x = np.clip(x, -1.0, 1.0)
+    return 0.5 + (x * np.sqrt(1.0 - x * x) + np.arcsin(x)) / np.pi
+
+
+def test_haarstart_draw_order_is_the_reference_normal_stream(oracle, golden):
+    z = golden["normals_seed13526463_1024"].reshape(-1, 4)
+    U = oracle.haarstart(SU2, D4, SEED)[: z.shape[0]]
+    want = z / np.sqrt((z * z).sum(1))[:, None]
+    np.testing.assert_allclose(U, want, rtol=0, atol=1e-15)
+
+
+@pytest.fixture(scope="module")
+def haar16(oracle):
+    return oracle.haarstart(SU2, (16, 16, 16, 16, 4), 20240601)
+
+
+def test_haarstart_moments(haar16):
+    q = haar16
+    n = q.shape[0]
+    np.testing.assert_allclose((q * q).sum(1), 1.0, rtol=0, atol=1e-15)
+    # uniform on S^3 in R^4: E[x^2] = 1/4, E[x^4] = 3/(4*6) = 1/8, E[x^8] = 105/1920
+    sd2 = math.sqrt(1.0 / 8 - 1.0 / 16) / math.sqrt(n)
+    sd4 = math.sqrt(105.0 / 1920 - 1.0 / 64) / math.sqrt(n)
+    for k in range(4):
+        assert abs(q[:, k].mean()) < 6 * 0.5 / math.sqrt(n)
+        assert abs((q[:, k] ** 2).mean() - 0.25) < 6 * sd2
+        assert abs((q[:, k] ** 4).mean() - 0.125) < 6 * sd4
+    # components of one link are uncorrelated
+    c = np.corrcoef(q.T)
+    assert np.abs(c - np.eye(4)).max() < 6 / math.sqrt(n)
+
+
+def test_haarstart_component_law_ks(haar16):
+    from scipy import stats
+    for k in range(4):
+        assert stats.kstest(haar16[:, k], _semicircle_cdf).pvalue > 1e-4
+
+
+def test_haarstart_left_invariance(oracle, haar16):
+    """g U has the law of U for a fixed g (the reference's own product,
+    su2.hh:68-84, through the oracle's known-answer-pinned su2_mul)."""
+    from scipy import stats
+    g = np.array([0.3, -0.5, 0.7, 0.1])
+    g /= np.linalg.norm(g)
+    q = haar16[:65536]
+    gq = np.array([oracle.su2_mul(g, x) for x in q])
+    for k in range(4):
+        assert stats.kstest(gq[:, k], _semicircle_cdf).pvalue > 1e-4
+        assert stats.ks_2samp(gq[:, k], haar16[65536:, k]).pvalue > 1e-4
+
+
+def test_haarstart_links_independent(haar16):
+    q = haar16
+    n = q.shape[0] // 2
+    for k in range(4):
+        assert abs(np.corrcoef(q[0::2, k], q[1::2, k])[0, 1]) < 6 / math.sqrt(n)
+
+
+def test_haarstart_checks_reject_planted_mistakes(oracle):
+    """The moment and KS checks above fail for the plausible wrong recipes:
+    the reference's random_element start, and normalised uniform components."""
+    from scipy import stats
+    U = oracle.hotstart(SU2, (16, 16, 16, 16, 4), 20240601, 1.0)  # random_element recipe
+    assert abs((U[:, 0] ** 2).mean() - 0.25) > 0.1
+    assert stats.kstest(U[:, 0], _semicircle_cdf).pvalue < 1e-10
+    rng = np.random.default_rng(1)
+    c = rng.uniform(-1, 1, size=(262144, 4))
+    c /= np.linalg.norm(c, axis=1)[:, None]
+    n = c.shape[0]
+    sd4 = math.sqrt(105.0 / 1920 - 1.0 / 64) / math.sqrt(n)
+    assert abs((c[:, 0] ** 4).mean() - 0.125) > 6 * sd4
+
+
+def test_haarstart_u1_uniform_phase(oracle):
+    from scipy import stats
+    u = oracle.haarstart(U1, (64, 1, 1, 64, 2), SEED)
+    assert u.min() >= -math.pi and u.max() < math.pi
+    assert stats.kstest(u, "uniform", args=(-math.pi, 2 * math.pi)).pvalue > 1e-4
