@@ -350,6 +350,11 @@ def measured_traffic(L, form):
     return None if rec is None else rec.get("hbm_bytes_per_launch")
 
 
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def kernel_name(prec, L):
     """Instantiation of the specialised step that a P-then-U launch of the
     L^4 SU(2) lattice takes (su2_kernels.hip launch_fast): template flags
@@ -403,10 +408,13 @@ def main():
 
     import su2_amd as S
     S.load_library()
+    progress("FP64 form")
     f64 = run_form(S, S.FP64, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
+    progress("FP32 form")
     f32 = run_form(S, S.FP32, args.L, args.nsteps, args.steps, args.warmup, args.seed, args.beta)
     ddh32 = ddh48 = None
     if not args.no_precision:
+        progress(f"FP32 vs FP64 dH at {args.L}^4")
         ddh32 = delta_dH(S, args.L, args.nsteps, args.beta, args.seed, args.ddh_traj,
                          nchain=args.chain)
     leg48 = None
@@ -414,15 +422,18 @@ def main():
         Lp = args.prec_L
         leg48 = {"lattice": [Lp] * 4, "beta": args.prec_beta}
         for name, prec in (("fp64", S.FP64), ("fp32", S.FP32)):
+            progress(f"{name} form at {Lp}^4")
             r = run_form(S, prec, Lp, args.nsteps, 3, 1, args.seed, args.prec_beta)
             leg48[name] = dict(r, roofline=form_roofline(r, prec, Lp),
                                traffic=measured_traffic(Lp, name))
         if not args.no_precision:
+            progress(f"FP32 vs FP64 dH at {Lp}^4")
             ddh48 = delta_dH(S, Lp, args.nsteps, args.prec_beta, args.seed, args.ddh_traj,
                              nchain=args.chain)
             leg48["delta_dH"] = ddh48
     copy_ms = S.copy_probe(1 << 30, 20)
     copy_gbs = 2.0 * (1 << 30) / (copy_ms * 1e-3) / 1e9
+    progress("batch, flow adjoint, Metropolis")
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
     flow = None if args.no_flow else flow_adjoint(S)
     metro = None if args.no_metropolis else metropolis(S, args.L)
@@ -439,6 +450,7 @@ def main():
         scan.append(e)
     cpu = None
     if not args.no_cpu:
+        progress("CPU baseline")
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
     n = args.nsteps
     roof = form_roofline(f64, 0, args.L)

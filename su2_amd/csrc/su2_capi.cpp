@@ -24,6 +24,7 @@
 #include "../../include/su2_mi355x.h"
 #include "../../include/su2_mi355x/normals.hh"
 #include "su2_launch.hpp"
+#include "su2_handle.hpp"
 
 namespace {
 
@@ -65,6 +66,9 @@ struct su2gpu_lattice {
   long nlinks = 0;
   size_t real_bytes = 8, U_bytes = 0, P_bytes = 0;
   void *U[NBUF] = {nullptr, nullptr, nullptr, nullptr};
+  // 1: U[i] holds FP64 SU(2) links in the half-field layout (an integrator's
+  // intermediate links); 0: the record layout every other path reads
+  int half[NBUF] = {0, 0, 0, 0};
   int cur = 0;
   void *P = nullptr, *deriv = nullptr;
   double *partials = nullptr;
@@ -237,8 +241,11 @@ int fused_ptr(L *h, const void *Uin, void *Uout, void *P, su2k::FusedCoef c, int
   return last_launch("md_fused");
 }
 
-// fused "P then U" (Uout >= 0) or momentum-only (Uout < 0) step
-int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row) {
+// fused "P then U" (Uout >= 0) or momentum-only (Uout < 0) step.  allow_half
+// (the integrators only): where the step supports it, the new links go out in
+// the half-field layout; links already in it are read in it.
+int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row,
+          bool allow_half = false) {
   su2k::FusedLaunch fl;
   fl.Uin = h->U[uin];
   fl.Uout = uout >= 0 ? h->U[uout] : nullptr;
@@ -251,11 +258,41 @@ int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row) {
     fl.c.flags |= su2k::F_ACCUM;
     fl.partials = row(h, accum_row);
   }
+  fl.in_half = h->half[uin];
+  if (allow_half && su2k::fused_half_ok(h->group, h->prec, h->geo, fl.c.flags))
+    fl.out_half = uout >= 0 ? 1 : 0;
+  else if (fl.in_half)
+    return fail(SU2GPU_ESTATE, "links in the half-field layout where the step cannot read them");
+  bool ok;
   {
     Prof pr(h, uout >= 0 ? "md_fused" : "md_p_only");
-    su2k::launch_fused(h->group, h->prec, fl, h->stream);
+    ok = su2k::launch_fused(h->group, h->prec, fl, h->stream);
   }
+  if (!ok) return fail(SU2GPU_ESTATE, "fused step: link layout not supported for this lattice");
+  if (uout >= 0) h->half[uout] = fl.out_half;
   return last_launch("md_fused");
+}
+
+// links of buffer b in the record layout: a half-field buffer is converted
+// into a free buffer (not one of keep), which becomes the new b
+int to_record(L *h, int *b, std::initializer_list<int> keep) {
+  if (!h->half[*b]) return SU2GPU_OK;
+  int out = -1;
+  for (int i = 0; i < NBUF && out < 0; ++i) {
+    bool used = (i == *b);
+    for (int k : keep) used = used || (k == i);
+    if (!used) out = i;
+  }
+  if (out < 0) return fail(SU2GPU_ESTATE, "no free link buffer");
+  CK(ensure_buf(h, out));
+  {
+    Prof pr(h, "convert");
+    su2k::launch_half_to_record(h->U[*b], h->U[out], h->nlinks, h->stream);
+  }
+  CK(last_launch("half_to_record"));
+  h->half[out] = 0;
+  *b = out;
+  return SU2GPU_OK;
 }
 
 // one reduce launch for every row of rl
@@ -292,9 +329,12 @@ int plaq_launch(L *h, int uin, int uout, double w_t, double w_s, int mu_start, i
   {
     Prof pr(h, "plaq");
     nb = su2k::launch_plaq(h->group, h->prec, h->U[uin], uout >= 0 ? h->U[uout] : nullptr,
-                           h->geo, w_t, w_s, mu_start, flags, row(h, r), h->stream);
+                           h->geo, w_t, w_s, mu_start, flags, row(h, r), h->stream, h->half[uin]);
   }
+  if (nb < 0) return fail(SU2GPU_ESTATE, "plaquette sum: link layout not supported");
   CK(last_launch("plaq"));
+  // restored links (F_RESTORE) are written in the record layout
+  if (uout >= 0) h->half[uout] = 0;
   return reduce_row(h, r, nb, defer);
 }
 
@@ -432,24 +472,31 @@ int run_integrator(L *h, int kind, size_t nsteps, double tau, size_t exponent,
         CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
                         protect.size() > 1 ? *(protect.begin() + 1) : cur},
                     &out));
-        CK(fused(h, cur, out, h->P, c, acc_row));
+        CK(fused(h, cur, out, h->P, c, acc_row, true));
         cur = out;
         i += 2;
       } else {
         const int r = (accum && first) ? row_base + 0 : (accum && last_p) ? row_base + 3 : -1;
-        CK(fused(h, cur, -1, h->P, c, r));
+        CK(fused(h, cur, -1, h->P, c, r, true));
         if (accum && last_p && !first) io->last_p_only = true;
         i += 1;
       }
       if (first && accum) io->first_accum = true;
       first = false;
     } else {
+      // a link update without a momentum step before it (no integrator of
+      // integrator.hh has one; kept general): the generic kernel, record layout
+      {
+        std::initializer_list<int> keep = protect;
+        CK(to_record(h, &cur, keep));
+      }
       if (is_prot(cur)) {
         int out;
         CK(free_buf(h, {cur, protect.size() > 0 ? *protect.begin() : cur,
                         protect.size() > 1 ? *(protect.begin() + 1) : cur},
                     &out));
         HC(hipMemcpyAsync(h->U[out], h->U[cur], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+        h->half[out] = 0;
         cur = out;
       }
       {
@@ -518,6 +565,7 @@ int upload_links_packed(L *h, const double *src_dev_staging, int stride, int off
   int b;
   CK(ensure_buf(h, h->cur));
   b = h->cur;
+  h->half[b] = 0;
   {
     Prof pr(h, "convert");
     su2k::launch_links_from_aos(h->group, h->prec, src_dev_staging, stride, offset, h->U[b], h->V,
@@ -600,6 +648,24 @@ int su2gpu_internal_set_error(int code, const char *msg) {
   g_err = msg;
   return code;
 }
+
+}  // extern "C"
+
+// what the entry points of other translation units need from a handle
+// (declared in su2_handle.hpp; not part of the C ABI)
+int su2_internal_lattice_view(su2gpu_lattice *h, su2k::LatticeView *v) {
+  CK(check(h));
+  v->group = h->group;
+  v->prec = h->prec;
+  v->geo = h->geo;
+  v->U = h->U[h->cur];
+  v->stream = h->stream;
+  v->device = h->device;
+  v->pending = h->pending;
+  return SU2GPU_OK;
+}
+
+extern "C" {
 
 int su2gpu_device_count(int *n) {
   if (!n) return fail(SU2GPU_EINVAL, "null pointer");
@@ -966,7 +1032,9 @@ int su2gpu_integrate(su2gpu_lattice *h, int integrator, size_t nsteps, double ta
   if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
   IntegrateOut io;
   CK(run_integrator(h, integrator, nsteps, tau, exponent, {}, false, 0, &io));
-  h->cur = io.final_buf;
+  int fin = io.final_buf;
+  CK(to_record(h, &fin, {}));
+  h->cur = fin;
   if (restore) CK(su2gpu_restoreSU(h));
   return SU2GPU_OK;
 }
@@ -1199,6 +1267,7 @@ int su2gpu_gradient_flow(su2gpu_lattice *h, double tmax, double eps, double tsta
   int vt;
   CK(free_buf(h, {orig}, &vt));
   HC(hipMemcpyAsync(h->U[vt], h->U[orig], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+  h->half[vt] = 0;
   // ping-pong on the two buffers other than orig
   const int d = h->nd;
   const double ndf = d * (d - 1) / 2.0, ndf_ss = (d - 1) * (d - 2) / 2.0;
@@ -1425,6 +1494,7 @@ int su2gpu_flow_forward(su2gpu_lattice *h, double eps, size_t nsteps, size_t ckp
   int tmp;
   CK(free_buf(h, {save}, &tmp));
   HC(hipMemcpyAsync(h->U[tmp], f.W[a], h->U_bytes, hipMemcpyDeviceToDevice, h->stream));
+  h->half[tmp] = 0;
   h->cur = tmp;
   int rc;
   const double t = eps * (double)nsteps;

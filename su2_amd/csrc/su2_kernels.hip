@@ -117,6 +117,16 @@ __global__ __launch_bounds__(NT) void k_restore(typename G::link *__restrict__ U
   if (i < n) U[i] = G::restore(U[i]);
 }
 
+// FP64 SU(2) half-field layout (su2_md_fast.hpp QHalf) -> record layout
+__global__ __launch_bounds__(NT) void k_half_to_record(const su2_d2 *__restrict__ src,
+                                                       quat<double> *__restrict__ dst, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) {
+    const su2_d2 a = src[i], b = src[(size_t)n + i];
+    dst[i] = quat<double>{a.x, a.y, b.x, b.y};
+  }
+}
+
 // Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) computed in
 // FP64 whatever the storage precision.  With F_RESTORE each link is restored on
 // load and the restored links of the site are written to Uout (the end of an
@@ -463,11 +473,52 @@ static int nt_pref() {
   if (v == -2) v = env_int("SU2GPU_NT", -1);
   return v;
 }
+// SU2GPU_HALF = 0 keeps the FP64 trajectory's intermediate links in the record
+// layout instead of the half-field layout
+static int half_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_HALF", 1);
+  return v;
+}
 // SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
 static int zs_pref() {
   static int v = -1;
   if (v < 0) v = env_int("SU2GPU_ZS", 1);
   return v;
+}
+
+// The specialised step on FP64 links in the half-field layout: input UI
+// (QHalf or a record-layout pointer), output QHalfW (a P-only step passes a
+// null QHalfW it never writes)
+template <bool W, bool A, class UI>
+static void launch_half(bool zs, bool nt, UI uin, QHalfW uout, const FusedLaunch &L,
+                        const FastGeo &f, hipStream_t st) {
+  const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
+#define SU2_HALF(Z, N)                                                                          \
+  hipLaunchKernelGGL((k_md_su2_4d<double, W, A, 1, Z, false, N, UI, QHalfW>), grid, block, 0, st, \
+                     uin, uout, (double *)L.P, f, L.c, L.partials, L.pstride)
+  if (zs && nt) SU2_HALF(true, true);
+  else if (zs) SU2_HALF(true, false);
+  else if (nt) SU2_HALF(false, true);
+  else SU2_HALF(false, false);
+#undef SU2_HALF
+}
+
+// the conditions under which launch_fast takes the specialised thread-per-link
+// step for a launch (shared by launch_fast and fused_half_ok)
+static bool fast_geo_for(const Geo &g, FastGeo &f) {
+  const int xw = xwalk_pref();
+  const bool xwalk = xw >= 0 ? xw != 0 : (g.L[2] * g.L[3]) / NTF >= 16;
+  return make_fast_geo(g.L, g.V, tblock_pref() != 0, f, xwalk);
+}
+
+bool fused_half_ok(int group, int prec, const Geo &g, int flags) {
+  if (group != 1 || prec != 0 || g.nd != 4) return false;
+  if (fast_level() < 2 || fast_shape() == 1 || half_pref() == 0) return false;
+  if (flags & (F_PZERO | F_ROUND | F_ANISO)) return false;
+  if (fused_mode(g) == 0) return false;
+  FastGeo f;
+  return fast_geo_for(g, f);
 }
 
 template <class G, int ND>
@@ -483,9 +534,7 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     if (fused_mode(L.geo) == 0) return false;
     FastGeo f;
     const bool acc = (L.c.flags & F_ACCUM) != 0;
-    const int xw = xwalk_pref();
-    const bool xwalk = xw >= 0 ? xw != 0 : (L.geo.L[2] * L.geo.L[3]) / NTF >= 16;
-    if (!make_fast_geo(L.geo.L, L.geo.V, tblock_pref() != 0, f, xwalk)) {
+    if (!fast_geo_for(L.geo, f)) {
       // FP32: the same step body over the lattice's own geometry (FP64 keeps
       // the generic kernel there: 9.9 against 11.1 us at 12^4, 28.5 against
       // 29.8 us at 20^4; FP32 8.5 -> 8.1 and 17.6 -> 16.4 us)
@@ -522,6 +571,30 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     const int ntp = nt_pref();
     const double fields = 4.0 * L.geo.V * (2.0 * sizeof(quat<T>) + 3.0 * sizeof(T));
     const bool nt = ntp >= 0 ? ntp != 0 : fields > 256.0 * 1024 * 1024;
+    if (L.in_half || L.out_half) {
+      // the host asked for it only where fused_half_ok holds
+      if constexpr (sizeof(T) == 8) {
+        const size_t nl = 4 * (size_t)L.geo.V;
+        const su2_d2 *ia = (const su2_d2 *)L.Uin;
+        su2_d2 *oa = (su2_d2 *)L.Uout;
+        const QHalfW hout{oa, oa ? oa + nl : nullptr};
+        if (L.Uout && !L.out_half) return false;  // half -> record steps are not built
+        if (L.in_half) {
+          const QHalf hin{ia, ia + nl};
+          if (L.Uout) acc ? launch_half<true, true>(zs, nt, hin, hout, L, f, st)
+                          : launch_half<true, false>(zs, nt, hin, hout, L, f, st);
+          else acc ? launch_half<false, true>(zs, nt, hin, hout, L, f, st)
+                   : launch_half<false, false>(zs, nt, hin, hout, L, f, st);
+        } else {
+          const quat<double> *rin = (const quat<double> *)L.Uin;
+          acc ? launch_half<true, true>(zs, nt, rin, hout, L, f, st)
+              : launch_half<true, false>(zs, nt, rin, hout, L, f, st);
+        }
+        return true;
+      } else {
+        return false;
+      }
+    }
     dim3 grid((site ? 1 : 4) * (L.geo.V / NTF)), block(NTF);
 #define SU2_FAST(W, A)                                                                          \
   if (site)                                                                                     \
@@ -596,21 +669,25 @@ static bool launch_fast_pout(const FusedLaunch &L, hipStream_t st) {
 }
 
 template <class G, int ND>
-static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
+static bool launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
   using link = typename G::link;
   using T = typename G::real;
+  if (L.in_half || L.out_half) {
+    // only the specialised step reads / writes the half-field layout
+    if (L.Pout) return false;
+    return launch_fast<G, ND>(L, st);
+  }
   if (L.Pout) {
-    if (launch_fast_pout<G, ND>(L, st)) return;
+    if (launch_fast_pout<G, ND>(L, st)) return true;
     // elsewhere: copy the momenta, then the in-place step on the copy
     const size_t bytes = (size_t)L.geo.V * ND * G::NMOM * sizeof(T);
     (void)hipMemcpyAsync(L.Pout, L.P, bytes, hipMemcpyDeviceToDevice, st);
     FusedLaunch L2 = L;
     L2.P = L.Pout;
     L2.Pout = nullptr;
-    launch_fused_nd<G, ND>(L2, st);
-    return;
+    return launch_fused_nd<G, ND>(L2, st);
   }
-  if (launch_fast<G, ND>(L, st)) return;
+  if (launch_fast<G, ND>(L, st)) return true;
   const int mode = fused_mode(L.geo);
   if (mode != 0) {
     dim3 grid(ND * nblk(L.geo.V)), block(NT);
@@ -625,7 +702,7 @@ static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
       else SU2_LINK(false, false);
     }
 #undef SU2_LINK
-    return;
+    return true;
   }
   dim3 grid(nblk(L.geo.V)), block(NT);
   if (L.Uout)
@@ -634,24 +711,30 @@ static void launch_fused_nd(const FusedLaunch &L, hipStream_t st) {
   else
     hipLaunchKernelGGL((k_md_fused<G, ND, false>), grid, block, 0, st, (const link *)L.Uin,
                        (link *)nullptr, (T *)L.P, L.geo, L.c, L.partials, L.pstride);
+  return true;
 }
 
-template <class G> static void launch_fused_g(const FusedLaunch &L, hipStream_t st) {
+template <class G> static bool launch_fused_g(const FusedLaunch &L, hipStream_t st) {
   switch (L.geo.nd) {
-    case 2: launch_fused_nd<G, 2>(L, st); break;
-    case 3: launch_fused_nd<G, 3>(L, st); break;
-    default: launch_fused_nd<G, 4>(L, st); break;
+    case 2: return launch_fused_nd<G, 2>(L, st);
+    case 3: return launch_fused_nd<G, 3>(L, st);
+    default: return launch_fused_nd<G, 4>(L, st);
   }
 }
 
-void launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st) {
+bool launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st) {
   if (group == 1) {
-    if (prec == 0) launch_fused_g<GSU2<double>>(L, st);
-    else launch_fused_g<GSU2<float>>(L, st);
-  } else {
-    if (prec == 0) launch_fused_g<GU1<double>>(L, st);
-    else launch_fused_g<GU1<float>>(L, st);
+    if (prec == 0) return launch_fused_g<GSU2<double>>(L, st);
+    return launch_fused_g<GSU2<float>>(L, st);
   }
+  if (L.in_half || L.out_half) return false;
+  if (prec == 0) return launch_fused_g<GU1<double>>(L, st);
+  return launch_fused_g<GU1<float>>(L, st);
+}
+
+void launch_half_to_record(const void *src, void *dst, long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_half_to_record, dim3(nblk(n)), dim3(NT), 0, st, (const su2_d2 *)src,
+                     (quat<double> *)dst, (int)n);
 }
 
 int fused_blocks(const Geo &g) { return fused_mode(g) == 0 ? nblk(g.V) : g.nd * nblk(g.V); }
@@ -703,8 +786,25 @@ static void launch_plaq_g(const void *Uin, void *Uout, const Geo &g, double w_t,
   }
 }
 int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
-                double w_s, int mu_start, int flags, double *partials, hipStream_t st) {
+                double w_s, int mu_start, int flags, double *partials, hipStream_t st,
+                int in_half) {
   FastGeo f;
+  if (in_half) {
+    // FP64 SU(2) links in the half-field layout (the specialised geometry,
+    // fused_half_ok): restored links, if asked for, go out in the record layout
+    if (!(group == 1 && prec == 0 && make_fast_geo(g.L, g.V, tblock_pref() != 0, f))) return -1;
+    const dim3 grid(g.V / NTF), block(NTF);
+    const int an = (flags & F_ANISO) ? 1 : 0;
+    const size_t nl = 4 * (size_t)g.V;
+    const QHalf hin{(const su2_d2 *)Uin, (const su2_d2 *)Uin + nl};
+    if (flags & F_RESTORE)
+      hipLaunchKernelGGL((k_plaq_su2_4d<double, true, QHalf>), grid, block, 0, st, hin,
+                         (quat<double> *)Uout, f, w_t, w_s, mu_start, an, partials);
+    else
+      hipLaunchKernelGGL((k_plaq_su2_4d<double, false, QHalf>), grid, block, 0, st, hin,
+                         (quat<double> *)Uout, f, w_t, w_s, mu_start, an, partials);
+    return g.V / NTF;
+  }
   if (group == 1 && g.nd == 4 && fast_level() > 0 &&
       make_fast_geo(g.L, g.V, tblock_pref() != 0, f)) {
     // specialised SU(2) d=4 sum: one thread per site, V / NTF blocks

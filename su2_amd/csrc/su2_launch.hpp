@@ -43,18 +43,31 @@ struct FusedLaunch {
   FusedCoef c;
   double *partials;  // 3 rows of fused_blocks(geo) doubles (F_ACCUM) ...
   int pstride;       // ... pstride doubles apart
+  // FP64 SU(2) half-field layout (su2_md_fast.hpp QHalf) of the links read /
+  // written; only where fused_half_ok says the step supports it
+  int in_half = 0, out_half = 0;
 };
 
-void launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st);
+// false if a half-field layout was asked for where the step does not support it
+bool launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st);
+// whether a fused launch of this lattice with these flags runs the specialised
+// step that reads and writes the half-field layout (FP64 SU(2) d = 4,
+// isotropic, specialised launch layout)
+bool fused_half_ok(int group, int prec, const Geo &g, int flags);
+// half-field links (n links) -> record layout, out-of-place
+void launch_half_to_record(const void *src, void *dst, long n, hipStream_t st);
 int fused_blocks(const Geo &g);  // blocks (= partial-sum entries) of one fused launch
 int site_blocks(const Geo &g);   // blocks of the one-thread-per-site sum kernels
 int fused_mode(const Geo &g);    // 0 thread per site, 1 thread per link, 2 link + z shuffles
 void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int nd, double eps,
                          double round_n, int flags, hipStream_t st);
 void launch_restore(int group, int prec, void *U, int n, hipStream_t st);
-// returns the number of block partials written (one row, at most fused_blocks(g))
+// returns the number of block partials written (one row, at most fused_blocks(g));
+// in_half: Uin in the half-field layout (where fused_half_ok holds), Uout always
+// in the record layout
 int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
-                double w_s, int mu_start, int flags, double *partials, hipStream_t st);
+                double w_s, int mu_start, int flags, double *partials, hipStream_t st,
+                int in_half = 0);
 int sumsq_blocks(long n);
 void launch_sumsq(int prec, const void *P, long n, double *partials, hipStream_t st);
 void launch_scale(int prec, void *P, long n, double f, hipStream_t st);

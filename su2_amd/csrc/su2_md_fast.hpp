@@ -197,6 +197,51 @@ template <bool NT, typename E> __device__ __forceinline__ void sto_h(E *__restri
   else sto(base, i, v);
 }
 
+// Half-field layout of an FP64 SU(2) link field of n links: the 16-B halves
+// (Re a, Im a) of every link first, then the halves (Re b, Im b), each in the
+// link order of the record layout (link (s, mu) at mu V + s).  A wave's load
+// of 64 consecutive links is then two contiguous 1 KB runs instead of two
+// instructions that each touch every other 16 B of 2 KB.  The MD steps of a
+// trajectory keep their intermediate links in it (QHalf / QHalfW: the two
+// half-field bases, already offset to a direction plane where one is taken);
+// the start and final links stay in the record layout.
+struct QHalf {
+  const su2_d2 *a, *b;
+};
+struct QHalfW {
+  su2_d2 *a, *b;
+};
+template <typename T> __device__ __forceinline__ const quat<T> *fld_add(const quat<T> *p, size_t k) {
+  return p + k;
+}
+template <typename T> __device__ __forceinline__ quat<T> *fld_add(quat<T> *p, size_t k) {
+  return p + k;
+}
+__device__ __forceinline__ QHalf fld_add(QHalf f, size_t k) { return QHalf{f.a + k, f.b + k}; }
+__device__ __forceinline__ QHalfW fld_add(QHalfW f, size_t k) { return QHalfW{f.a + k, f.b + k}; }
+template <typename T> __device__ __forceinline__ quat<T> ldq(const quat<T> *__restrict__ p, uint32_t i) {
+  return ldo(p, i);
+}
+__device__ __forceinline__ quat<double> ldq(QHalf f, uint32_t i) {
+  const su2_d2 x = ldo(f.a, i), y = ldo(f.b, i);
+  return quat<double>{x.x, x.y, y.x, y.y};
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void stq(quat<T> *__restrict__ p, uint32_t i, const quat<T> &v) {
+  sto_h<NT>(p, i, v);
+}
+template <bool NT>
+__device__ __forceinline__ void stq(QHalfW f, uint32_t i, const quat<double> &v) {
+  const su2_d2 x = {v.ar, v.ai}, y = {v.br, v.bi};
+  if constexpr (NT) {
+    __builtin_nontemporal_store(x, (su2_d2 *)((char *)f.a + i * 16u));
+    __builtin_nontemporal_store(y, (su2_d2 *)((char *)f.b + i * 16u));
+  } else {
+    sto(f.a, i, x);
+    sto(f.b, i, y);
+  }
+}
+
 // u v as four FMA chains (16 FMA / MUL), conjugations of the operands passed in
 // through qdag and folded into the source negations of the FMAs
 template <typename T> __device__ __forceinline__ quat<T> qmul_fma(const quat<T> &u, const quat<T> &v) {
@@ -231,11 +276,11 @@ __device__ __forceinline__ void acc_dw(double (&K)[4], double w, const quat<T> &
 // Momentum and link update of one link from its staple sum K (FP64):
 // M = U K, P <- P - ep coef 2 Im(M) rounded once, U' = exp(eu P) U.
 // Returns Re(M) and the new momentum for the block sums.
-template <typename T, bool WRITE_U, bool NT = false>
+template <typename T, bool WRITE_U, bool NT = false, class UO = quat<T> *>
 __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (&K)[4], T pi0,
                                                 T pi1, T pi2, T *__restrict__ P0,
                                                 T *__restrict__ P1, T *__restrict__ P2,
-                                                quat<T> *__restrict__ Uo, uint32_t s,
+                                                UO Uo, uint32_t s,
                                                 const FusedCoef &c, T (&pn)[3]) {
   const double ua = u.ar, ub = u.ai, uc = u.br, ud = u.bi;
   const double Mai = ua * K[1] + ub * K[0] - ud * K[2] + uc * K[3];
@@ -280,7 +325,7 @@ __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (
       un = GSU2<T>::expmul(pn, T(c.eu), u);
       if (c.flags & F_RESTORE) un = GSU2<T>::restore(un);
     }
-    sto_h<NT>(Uo, s, un);
+    stq<NT>(Uo, s, un);
   }
   return ua * K[0] - ub * K[1] - uc * K[2] - ud * K[3];
 }
@@ -322,9 +367,8 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat
 // the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it reads
 // g.V and, with ZS, g.Lz (the extent of direction ND - 1).
 template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false, int ND = 4,
-          bool NT = false, bool POUT = false>
-__device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
-                                            quat<T> *__restrict__ Uout, T *__restrict__ P,
+          bool NT = false, bool POUT = false, class UI = const quat<T> *, class UO = quat<T> *>
+__device__ __forceinline__ void md_su2_core(UI Uin, UO Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
                                             const FastSite &fs, T (&pi)[3], T (&pn)[3],
                                             double &mar, T *__restrict__ Pw = nullptr) {
@@ -332,11 +376,11 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
   const int s = fs.s;
   const int *up = fs.up, *dn = fs.dn;
   const int V = g.V;
-  const quat<T> *__restrict__ Um = Uin + (size_t)mu * V;
+  const UI Um = fld_add(Uin, (size_t)mu * V);
   T *__restrict__ P0 = P + (size_t)(mu * 3 + 0) * V;
   T *__restrict__ P1 = P + (size_t)(mu * 3 + 1) * V;
   T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
-  const quat<T> u = ldo(Um, (uint32_t)s);
+  const quat<T> u = ldq(Um, (uint32_t)s);
   const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
   // -0.0 is the identity of FP64 addition (+0.0 is not: +0.0 + -0.0 = +0.0),
   // so the first staple's addition folds away
@@ -356,22 +400,22 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
 #pragma unroll
     for (int k = 0; k < ND - 1; ++k) {
       const int nu = k + (k >= MU ? 1 : 0);
-      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
+      const UI Un = fld_add(Uin, (size_t)nu * V);
       if (ZS && nu == ND - 1) {
-        A[k] = ldo(Un, (uint32_t)(s + up[MU]));         // U_z(s+mu)
-        C[k] = ldo(Un, (uint32_t)s);                    // U_z(s)
+        A[k] = ldq(Un, (uint32_t)(s + up[MU]));         // U_z(s+mu)
+        C[k] = ldq(Un, (uint32_t)s);                    // U_z(s)
       } else if (ZS && MU == ND - 1) {
-        B[k] = ldo(Um, (uint32_t)(s + up[nu]));         // U_z(s+nu)
-        C[k] = ldo(Un, (uint32_t)s);                    // U_nu(s)
-        E[k] = ldo(Um, (uint32_t)(s + dn[nu]));         // U_z(s-nu)
-        F[k] = ldo(Un, (uint32_t)(s + dn[nu]));         // U_nu(s-nu)
+        B[k] = ldq(Um, (uint32_t)(s + up[nu]));         // U_z(s+nu)
+        C[k] = ldq(Un, (uint32_t)s);                    // U_nu(s)
+        E[k] = ldq(Um, (uint32_t)(s + dn[nu]));         // U_z(s-nu)
+        F[k] = ldq(Un, (uint32_t)(s + dn[nu]));         // U_nu(s-nu)
       } else {
-        A[k] = ldo(Un, (uint32_t)(s + up[MU]));           // U_nu(s+mu)
-        B[k] = ldo(Um, (uint32_t)(s + up[nu]));           // U_mu(s+nu)
-        C[k] = ldo(Un, (uint32_t)s);                      // U_nu(s)
-        D[k] = ldo(Un, (uint32_t)(s + up[MU] + dn[nu]));  // U_nu(s+mu-nu)
-        E[k] = ldo(Um, (uint32_t)(s + dn[nu]));           // U_mu(s-nu)
-        F[k] = ldo(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
+        A[k] = ldq(Un, (uint32_t)(s + up[MU]));           // U_nu(s+mu)
+        B[k] = ldq(Um, (uint32_t)(s + up[nu]));           // U_mu(s+nu)
+        C[k] = ldq(Un, (uint32_t)s);                      // U_nu(s)
+        D[k] = ldq(Un, (uint32_t)(s + up[MU] + dn[nu]));  // U_nu(s+mu-nu)
+        E[k] = ldq(Um, (uint32_t)(s + dn[nu]));           // U_mu(s-nu)
+        F[k] = ldq(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
       }
     }
     if (ZS) {
@@ -399,11 +443,11 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
 #pragma unroll
     for (int nu = 0; nu < ND; ++nu) {
       if (nu == mu) continue;
-      const quat<T> *__restrict__ Un = Uin + (size_t)nu * V;
-      staple_pair<T, AN>(K, (mu == 0 || nu == 0) ? c.w_t : c.w_s, ldo(Un, (uint32_t)(s + up[mu])),
-                  ldo(Um, (uint32_t)(s + up[nu])),
-                  ldo(Un, (uint32_t)s), ldo(Un, (uint32_t)(s + up[mu] + dn[nu])),
-                  ldo(Um, (uint32_t)(s + dn[nu])), ldo(Un, (uint32_t)(s + dn[nu])));
+      const UI Un = fld_add(Uin, (size_t)nu * V);
+      staple_pair<T, AN>(K, (mu == 0 || nu == 0) ? c.w_t : c.w_s, ldq(Un, (uint32_t)(s + up[mu])),
+                  ldq(Um, (uint32_t)(s + up[nu])),
+                  ldq(Un, (uint32_t)s), ldq(Un, (uint32_t)(s + up[mu] + dn[nu])),
+                  ldq(Um, (uint32_t)(s + dn[nu])), ldq(Un, (uint32_t)(s + dn[nu])));
     }
   }
   pi[0] = pi0;
@@ -411,23 +455,21 @@ __device__ __forceinline__ void md_su2_core(const quat<T> *__restrict__ Uin,
   pi[2] = pi2;
   // POUT: the new momentum goes to Pw (same layout), P is only read
   T *__restrict__ Q = POUT ? Pw : P;
-  mar = md_su2_update<T, WRITE_U, NT>(u, K, pi0, pi1, pi2, Q + (size_t)(mu * 3 + 0) * V,
-                                      Q + (size_t)(mu * 3 + 1) * V, Q + (size_t)(mu * 3 + 2) * V,
-                                      Uout + (size_t)mu * V,
-                                  (uint32_t)s, c, pn);
+  mar = md_su2_update<T, WRITE_U, NT, UO>(u, K, pi0, pi1, pi2, Q + (size_t)(mu * 3 + 0) * V,
+                                          Q + (size_t)(mu * 3 + 1) * V, Q + (size_t)(mu * 3 + 2) * V,
+                                          fld_add(Uout, (size_t)mu * V), (uint32_t)s, c, pn);
 }
 
 template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false,
-          bool NT = false, bool POUT = false>
-__device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
-                                            quat<T> *__restrict__ Uout, T *__restrict__ P,
+          bool NT = false, bool POUT = false, class UI = const quat<T> *, class UO = quat<T> *>
+__device__ __forceinline__ void md_su2_body(UI Uin, UO Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
                                             const FastSite &fs, double *__restrict__ partials,
                                             int pstride, double *lds,
                                             T *__restrict__ Pw = nullptr) {
   T pi[3], pn[3];
   double mar;
-  md_su2_core<T, MU, WRITE_U, ZS, AN, 4, NT, POUT>(Uin, Uout, P, g, c, fs, pi, pn, mar, Pw);
+  md_su2_core<T, MU, WRITE_U, ZS, AN, 4, NT, POUT, UI, UO>(Uin, Uout, P, g, c, fs, pi, pn, mar, Pw);
   if (ACCUM) {
     double r0 = sq3(pi[0], pi[1], pi[2]), r1 = 2.0 * mar, r2 = sq3(pn[0], pn[1], pn[2]);
     block_sum3<NTF>(r0, r1, r2, lds);
@@ -443,23 +485,24 @@ __device__ __forceinline__ void md_su2_body(const quat<T> *__restrict__ Uin,
 // time (mu is uniform per block); 0 = one body with mu at run time
 // ZS (with SPEC; needs 64 % Lz == 0): +-z neighbours by lane exchange
 // AN: anisotropic weights w_t / w_s on the planes (FMA accumulation)
+// UI / UO: the link field types read and written (record-layout pointers, or
+// QHalf / QHalfW for the FP64 half-field layout)
 template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false, bool AN = false,
-          bool NT = false>
-__global__ __launch_bounds__(NTF) void k_md_su2_4d(const quat<T> *__restrict__ Uin,
-                                                 quat<T> *__restrict__ Uout,
+          bool NT = false, class UI = const quat<T> *, class UO = quat<T> *>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d(UI Uin, UO Uout,
                                                  T *__restrict__ P, FastGeo g, FusedCoef c,
                                                  double *__restrict__ partials, int pstride) {
   __shared__ double lds[3 * NTF / 64];
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
   if (SPEC) {
     switch (fs.mu) {
-      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
-      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
+      default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
     }
   } else {
-    md_su2_body<T, -1, WRITE_U, ACCUM, false, AN, NT>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
+    md_su2_body<T, -1, WRITE_U, ACCUM, false, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
   }
 }
 
@@ -649,8 +692,10 @@ __global__ __launch_bounds__(NTF) void k_md_su2_geo(const quat<T> *__restrict__ 
 // Planes with mu < mu_start are skipped; aniso weights w_t (mu = 0) / w_s.
 // (Measured at 32^4 FP32: 19 us for this shape against 43 us for one thread
 // per link with one or two planes each.)
-template <typename T, bool RESTORE>
-__global__ __launch_bounds__(NTF) void k_plaq_su2_4d(const quat<T> *__restrict__ Uin,
+// UI: the input field's type (record layout or QHalf); Uout (RESTORE) is in
+// the record layout
+template <typename T, bool RESTORE, class UI = const quat<T> *>
+__global__ __launch_bounds__(NTF) void k_plaq_su2_4d(UI Uin,
                                                    quat<T> *__restrict__ Uout, FastGeo g,
                                                    double w_t, double w_s, int mu_start,
                                                    int aniso, double *__restrict__ partials) {
@@ -661,7 +706,7 @@ __global__ __launch_bounds__(NTF) void k_plaq_su2_4d(const quat<T> *__restrict__
   quat<double> own[4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    quat<T> q = ldo(Uin + (size_t)m * V, (uint32_t)s);
+    quat<T> q = ldq(fld_add(Uin, (size_t)m * V), (uint32_t)s);
     if (RESTORE) {
       q = qrestore(q);
       sto(Uout + (size_t)m * V, (uint32_t)s, q);
@@ -674,8 +719,8 @@ __global__ __launch_bounds__(NTF) void k_plaq_su2_4d(const quat<T> *__restrict__
     if (mu < mu_start) continue;
 #pragma unroll
     for (int nu = mu + 1; nu < 4; ++nu) {
-      quat<T> a = ldo(Uin + (size_t)nu * V, (uint32_t)(s + up[mu]));  // U_nu(s+mu)
-      quat<T> b = ldo(Uin + (size_t)mu * V, (uint32_t)(s + up[nu]));  // U_mu(s+nu)
+      quat<T> a = ldq(fld_add(Uin, (size_t)nu * V), (uint32_t)(s + up[mu]));  // U_nu(s+mu)
+      quat<T> b = ldq(fld_add(Uin, (size_t)mu * V), (uint32_t)(s + up[nu]));  // U_mu(s+nu)
       if (RESTORE) {
         a = qrestore(a);
         b = qrestore(b);
