@@ -173,8 +173,10 @@ int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, doub
  * n_hit Metropolis hits on every link, R = random_element(delta)
  * (random_element.hh:9-34), dS = beta/Nc (ReTr(U K) - ReTr(U R K)), accept if
  * dS < 0 or uniform < exp(-dS), U <- restoreSU(U R); anisotropic when xi != 1.
- * The links are visited per direction mu and site parity (every extent in use
- * must be even), and the random numbers come from one Philox4x32-10 block per
+ * The links are visited per direction mu and colour class (the site parity
+ * where every extent in use is even, else one of three classes of
+ * sum_{nu != mu} h(x_nu) mod 3, a proper colouring for any extents), and the
+ * random numbers come from one Philox4x32-10 block per
  * (link, hit) with key = seed and counter (link, hit, sweep): a sweep depends on
  * (seed, sweep) only and is not the reference's per-thread mt19937 stream.
  * *rate = accepted hits / (n_hit * links), *rate_time the same for the

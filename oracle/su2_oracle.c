@@ -1045,25 +1045,58 @@ static cplx staples_u1(const geom *g, const double *U, size_t s, int mu, double 
  * (s nd + mu, hit, sweep_lo, sweep_hi) and key (seed_lo, seed_hi) (alpha, u,
  * theta, acceptance; U(1): phase, -, -, acceptance), dS = beta/Nc (ReTr(U K) -
  * ReTr(U R K)), accept if dS < 0 or v < exp(-dS), U <- restoreSU(U R).  Links
- * are visited per direction mu, then site parity (sum of coordinates mod 2),
- * as the device sweep does; inside one (mu, parity) set the links are
- * independent, so the site order does not matter.  rates = {accepted / (N_hit
+ * are visited per direction mu, then colour class (sweep_colour: the site
+ * parity where every extent is even), as the device sweep does; inside one
+ * (mu, class) set the links are independent, so the site order does not
+ * matter.  rates = {accepted / (N_hit
  * links), accepted temporal / (N_hit V)} (flat-sweep.hh:160-163). */
+/* Independent sets of the sweep.  The staple of U_mu(x) holds, of the links
+ * of direction mu, only U_mu(x +- nu) with nu != mu, so the links U_mu(x) of
+ * one colour class of any proper colouring of the sites under the nu != mu
+ * neighbour relation can be updated at once.  All extents in use even: the
+ * two classes of the site parity (sum of the coordinates mod 2).  Otherwise
+ * three classes, c(x) = sum_{nu != mu} h(x_nu) mod 3 with
+ * h(k) = k + min(k, a) and a = (3 - L_nu mod 3) mod 3: each step along nu
+ * changes h by 1 or 2 (mod 3), the wrap from L_nu - 1 to 0 by 1, so
+ * neighbours never share a class. */
+static int sweep_ncolours(const geom *g) {
+  for (int d = 0; d < g->nd; ++d)
+    if (g->L[d] % 2) return 3;
+  return 2;
+}
+static int sweep_colour(const geom *g, const long c[4], int mu, int ncol) {
+  if (ncol == 2) return (int)((c[0] + c[1] + c[2] + c[3]) & 1);
+  long col = 0;
+  for (int d = 0; d < g->nd; ++d) {
+    if (d == mu) continue;
+    const long a = (3 - g->L[d] % 3) % 3, k = c[d];
+    col += k + (k < a ? k : a);
+  }
+  return (int)(col % 3);
+}
+
+ORC_API int orc_sweep_colour(const int *dims, const int *coords, int mu) {
+  geom g = mkgeom(dims);
+  const long c[4] = {coords[0], coords[1], coords[2], coords[3]};
+  return sweep_colour(&g, c, mu, sweep_ncolours(&g));
+}
+
 ORC_API void orc_sweep(int grp, const int *dims, double *U, double beta, double xi, uint64_t seed,
                        uint64_t sweep, double delta, long n_hit, double *rates) {
   geom g = mkgeom(dims);
   const int aniso = (xi != 1.0);
   const double Nc = (grp == 1) ? 2.0 : 1.0;
   const double beta_nc = beta / Nc;
+  const int ncol = sweep_ncolours(&g);
   double acc_all = 0, acc_t = 0;
   for (int mu = 0; mu < g.nd; ++mu) {
-    for (int par = 0; par < 2; ++par) {
+    for (int par = 0; par < ncol; ++par) {
       double acc_mu = 0;
 #pragma omp parallel for schedule(static) reduction(+ : acc_mu)
       for (size_t s = 0; s < g.V; ++s) {
         long c[4];
         coords_of(&g, s, c);
-        if (((c[0] + c[1] + c[2] + c[3]) & 1) != par) continue;
+        if (sweep_colour(&g, c, mu, ncol) != par) continue;
         const uint32_t lid = (uint32_t)lk(&g, s, mu);
         if (grp == 1) {
           const su2 K = staples_su2(&g, U, s, mu, xi, aniso);

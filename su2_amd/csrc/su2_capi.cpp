@@ -52,7 +52,7 @@ int fail(int code, const char *fmt, ...) {
   } while (0)
 
 constexpr int NBUF = 4;
-constexpr int NROWS = 32;
+constexpr int NROWS = 36;
 
 }  // namespace
 
@@ -93,6 +93,11 @@ struct su2gpu_lattice {
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
   std::map<std::string, std::pair<double, long>> prof_acc;
+  // Metropolis sweep on lattices with an odd extent: per direction mu the
+  // sites of each of the three colour classes (sweep_colour), on the device,
+  // class (mu, c) at sweep_list[sweep_off[3 mu + c] ..)
+  int *sweep_list = nullptr;
+  std::vector<int> sweep_off;
   // gradient-flow forward state kept for the backward pass
   struct Flow {
     bool valid = false;
@@ -768,6 +773,7 @@ int su2gpu_destroy(su2gpu_lattice *h) {
   if (h->hsums) hipHostFree(h->hsums);
   if (h->staging) hipFree(h->staging);
   if (h->hstage) hipHostFree(h->hstage);
+  if (h->sweep_list) hipFree(h->sweep_list);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return SU2GPU_OK;
@@ -1192,42 +1198,98 @@ int su2gpu_energy_density(su2gpu_lattice *h, int clover, int ss, double *E, doub
   return SU2GPU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// colour class of site (t, x, y, z) for the links of direction mu on a lattice
+// with an odd extent: sum over nu != mu of h(x_nu) mod 3, h(k) = k + min(k, a),
+// a = (3 - L_nu mod 3) mod 3 (each step along nu changes h by 1 or 2 mod 3
+// and the periodic wrap by 1, so neighbours never share a class)
+int sweep_colour3(const su2k::Geo &g, const int c[4], int mu) {
+  int col = 0;
+  for (int d = 0; d < g.nd; ++d) {
+    if (d == mu) continue;
+    const int a = (3 - g.L[d] % 3) % 3;
+    col += c[d] + (c[d] < a ? c[d] : a);
+  }
+  return col % 3;
+}
+
+int ensure_sweep_lists(L *h) {
+  if (h->sweep_list) return SU2GPU_OK;
+  const su2k::Geo &g = h->geo;
+  const int nd = h->nd, V = h->V;
+  std::vector<int> cnt((size_t)3 * nd, 0);
+  std::vector<int> col((size_t)nd * V);
+  int s = 0;
+  for (int t = 0; t < g.L[0]; ++t)
+    for (int x = 0; x < g.L[1]; ++x)
+      for (int y = 0; y < g.L[2]; ++y)
+        for (int z = 0; z < g.L[3]; ++z, ++s) {
+          const int c[4] = {t, x, y, z};
+          for (int mu = 0; mu < nd; ++mu) {
+            const int k = sweep_colour3(g, c, mu);
+            col[(size_t)mu * V + s] = k;
+            ++cnt[3 * mu + k];
+          }
+        }
+  h->sweep_off.assign((size_t)3 * nd + 1, 0);
+  for (int i = 0; i < 3 * nd; ++i) h->sweep_off[i + 1] = h->sweep_off[i] + cnt[i];
+  std::vector<int> list((size_t)nd * V), pos(h->sweep_off.begin(), h->sweep_off.end() - 1);
+  for (int mu = 0; mu < nd; ++mu)
+    for (int site = 0; site < V; ++site) list[pos[3 * mu + col[(size_t)mu * V + site]]++] = site;
+  HC(hipMalloc(&h->sweep_list, list.size() * sizeof(int)));
+  HC(hipMemcpy(h->sweep_list, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+  return SU2GPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int su2gpu_sweep(su2gpu_lattice *h, uint64_t seed, uint64_t sweep, double delta, size_t n_hit,
                  double *rate, double *rate_time) {
   CK(check(h));
   if (!rate || !rate_time) return fail(SU2GPU_EINVAL, "null pointer");
   if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending");
-  for (int mu = 0; mu < h->nd; ++mu)
-    if (h->geo.L[mu] % 2 != 0)
-      return fail(SU2GPU_EINVAL, "sweep: the extent along direction %d (%d) must be even", mu,
-                  h->geo.L[mu]);
   if (n_hit > (size_t)INT32_MAX) return fail(SU2GPU_EINVAL, "sweep: n_hit too large");
-  // one row of accepted-hit counts per sub-sweep (mu, parity), reduced together
+  // every extent in use even: the two parity classes; else three colour classes
+  int ncol = 2;
+  for (int mu = 0; mu < h->nd; ++mu)
+    if (h->geo.L[mu] % 2 != 0) ncol = 3;
+  if (ncol == 3) CK(ensure_sweep_lists(h));
+  // one row of accepted-hit counts per sub-sweep (mu, class), reduced together
   constexpr int R0 = 22;
-  static_assert(R0 + 8 <= NROWS, "partial rows");
-  const int nb = su2k::sweep_blocks(h->geo);
+  static_assert(R0 + 12 <= NROWS, "partial rows");
   const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
   su2k::RowList rl;
   for (int mu = 0; mu < h->nd; ++mu)
-    for (int par = 0; par < 2; ++par) {
+    for (int par = 0; par < ncol; ++par) {
+      const int r = R0 + ncol * mu + par;
+      const int *list = nullptr;
+      int nlist = -1;
+      if (ncol == 3) {
+        list = h->sweep_list + h->sweep_off[3 * mu + par];
+        nlist = h->sweep_off[3 * mu + par + 1] - h->sweep_off[3 * mu + par];
+      }
+      const int nb = su2k::sweep_blocks(h->geo, nlist);
+      if (nb == 0) continue;
       {
         Prof pr(h, "sweep");
-        su2k::launch_sweep(h->group, h->prec, h->U[h->cur], h->geo, mu, par, seed, sweep, delta,
-                           (int)n_hit, h->beta / Nc(h), w_t, w_s, h->aniso ? 1 : 0,
-                           row(h, R0 + 2 * mu + par), h->stream);
+        su2k::launch_sweep(h->group, h->prec, h->U[h->cur], h->geo, mu, par, list, nlist, seed,
+                           sweep, delta, (int)n_hit, h->beta / Nc(h), w_t, w_s, h->aniso ? 1 : 0,
+                           row(h, r), h->stream);
       }
       CK(last_launch("sweep"));
-      CK(reduce_row(h, R0 + 2 * mu + par, nb, &rl));
+      CK(reduce_row(h, r, nb, &rl));
     }
   CK(reduce_rows(h, rl));
   CK(fetch_sums(h));
   long double all = 0.0L, temporal = 0.0L;
-  for (int mu = 0; mu < h->nd; ++mu)
-    for (int par = 0; par < 2; ++par) {
-      const long double a = sum_of(h, R0 + 2 * mu + par);
-      all += a;
-      if (mu == 0) temporal += a;
-    }
+  for (int i = 0; i < rl.n; ++i) {
+    const long double a = sum_of(h, rl.row[i]);
+    all += a;
+    if (rl.row[i] < R0 + ncol) temporal += a;  // mu = 0
+  }
   const long double nh = n_hit > 0 ? (long double)n_hit : 1.0L;
   *rate = (double)(all / nh / (long double)h->nlinks);
   *rate_time = (double)(temporal / nh / (long double)h->V);

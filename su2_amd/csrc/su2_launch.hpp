@@ -74,7 +74,7 @@ void launch_scale(int prec, void *P, long n, double f, hipStream_t st);
 // rows to reduce in one launch: row k of the partials (stride row_stride
 // doubles) holds nb[k] block partials; its sum goes to out[2 row[k] + {0,1}]
 struct RowList {
-  static constexpr int MAX = 8;
+  static constexpr int MAX = 16;
   int n = 0;
   int row[MAX];
   int nb[MAX];
@@ -99,12 +99,14 @@ void launch_copy16(const void *src, void *dst, long n16, hipStream_t st);
 // gradient-flow adjoint (su2_flow.hip)
 void launch_flow_stage_adj(int prec, const void *Zs, void *lamW, void *lamZ, int V, int nd,
                            double c, int zero_lamZ, hipStream_t st);
-// Metropolis sub-sweep: the links U_mu(x), x of the given parity, n_hit hits
-// each; one accepted-hit count per block to partials[0 .. sweep_blocks(g))
-int sweep_blocks(const Geo &g);
-void launch_sweep(int group, int prec, void *U, const Geo &g, int mu, int parity, uint64_t seed,
-                  uint64_t sweep, double delta, int n_hit, double beta_nc, double w_t, double w_s,
-                  int aniso, double *partials, hipStream_t st);
+// Metropolis sub-sweep: the links U_mu(x), x of the given parity (list ==
+// nullptr, every extent even) or the nlist sites of a device site list, n_hit
+// hits each; one accepted-hit count per block to partials[0 .. sweep_blocks)
+int sweep_blocks(const Geo &g, int nlist);  // nlist < 0: the parity form
+void launch_sweep(int group, int prec, void *U, const Geo &g, int mu, int parity, const int *list,
+                  int nlist, uint64_t seed, uint64_t sweep, double delta, int n_hit,
+                  double beta_nc, double w_t, double w_s, int aniso, double *partials,
+                  hipStream_t st);
 // lamW += b (dD/dW)^T lamZ; R: scratch of 36 V reals (the b and c role
 // gradients of the 6 planes).  gather = false leaves the R terms to a later
 // launch_flow_gather or launch_flow_gather_stage_adj.

@@ -3,12 +3,15 @@
 //
 // Order.  The reference walks the even t-slices in parallel and, inside a
 // slice, the sites and directions one after another; a GPU needs independent
-// links.  For a fixed direction mu and site parity p = (sum of the
-// coordinates) mod 2, the links U_mu(x) with x of parity p are independent: the
-// staple of U_mu(x) holds only links of the other directions and U_mu(x +- nu)
-// of the other parity.  A sweep is the 2 nd launches (mu = 0..nd-1, p = 0, 1),
-// each over the V/2 sites of one parity; every extent in use must be even so
-// the parity is consistent across the periodic boundary.
+// links.  The staple of U_mu(x) holds, of the links of direction mu, only
+// U_mu(x +- nu) with nu != mu, so for a fixed mu the links of one class of a
+// proper colouring of the sites under those neighbour steps are independent.
+// Every extent in use even: the two classes of the site parity (sum of the
+// coordinates mod 2), 2 nd launches per sweep over V/2 sites each, sites
+// computed from the launch index.  Any odd extent: three classes
+// c(x) = sum_{nu != mu} h(x_nu) mod 3 with h(k) = k + min(k, a),
+// a = (3 - L_nu mod 3) mod 3 (every step along nu changes h by 1 or 2 mod 3,
+// the periodic wrap by 1), 3 nd launches over host-built site lists.
 //
 // One hit (flat-sweep.hh:95-110): R = random_element(delta)
 // (random_element.hh:9-25, :27-34), dS = beta/Nc (ReTr(U K) - ReTr(U R K)),
@@ -43,9 +46,12 @@ __device__ __forceinline__ double retr_mul_u1(double th, double kre, double kim)
   return c * kre - s * kim;
 }
 
+// list == nullptr: the sites of parity `parity` (every extent even);
+// otherwise the nlist sites list[0 .. nlist) of one colour class
 template <class G, int ND>
 __global__ __launch_bounds__(NTS) void k_sweep(typename G::link *__restrict__ U, Geo geo, int mu,
-                                               int parity, uint64_t seed, uint64_t sweep,
+                                               int parity, const int *__restrict__ list,
+                                               int nlist, uint64_t seed, uint64_t sweep,
                                                double delta, int n_hit, double beta_nc,
                                                double w_t, double w_s, int aniso,
                                                double *__restrict__ partials) {
@@ -54,22 +60,27 @@ __global__ __launch_bounds__(NTS) void k_sweep(typename G::link *__restrict__ U,
   using acc = typename G::acc;
   __shared__ double lds[NTS / 64];
   const int V = geo.V;
-  const int Lf = geo.L[ND - 1];  // fastest direction in use (even)
-  const int half = Lf / 2;
   const int j = blockIdx.x * NTS + threadIdx.x;
   double accepted = 0.0;
-  if (j < V / 2) {
-    // site j of this parity: row of the fastest direction, then every second site
-    const int row = j / half, k = j - row * half;
-    int rp = 0, q = row;
+  if (list ? j < nlist : j < V / 2) {
+    int s;
+    if (list) {
+      s = list[j];
+    } else {
+      // site j of this parity: row of the fastest direction, then every second site
+      const int Lf = geo.L[ND - 1];  // fastest direction in use (even)
+      const int half = Lf / 2;
+      const int row = j / half, k = j - row * half;
+      int rp = 0, q = row;
 #pragma unroll
-    for (int d = ND - 2; d >= 0; --d) {
-      const int Ld = geo.L[d];
-      const int qn = q / Ld;
-      rp += q - qn * Ld;
-      q = qn;
+      for (int d = ND - 2; d >= 0; --d) {
+        const int Ld = geo.L[d];
+        const int qn = q / Ld;
+        rp += q - qn * Ld;
+        q = qn;
+      }
+      s = row * Lf + 2 * k + ((parity + rp) & 1);
     }
-    const int s = row * Lf + 2 * k + ((parity + rp) & 1);
     int up[ND], dn[ND];
     neighbour_offsets<ND>(geo, s, up, dn);
     // staples (get_staples.hh:26-80): all upper, then all lower
@@ -134,48 +145,44 @@ __global__ __launch_bounds__(NTS) void k_sweep(typename G::link *__restrict__ U,
   if (threadIdx.x == 0) partials[blockIdx.x] = r;
 }
 
-int sweep_blocks(const Geo &g) { return (g.V / 2 + NTS - 1) / NTS; }
+int sweep_blocks(const Geo &g, int nlist) { return ((nlist >= 0 ? nlist : g.V / 2) + NTS - 1) / NTS; }
 
 template <class G>
-static void sweep_g(void *U, const Geo &g, int mu, int parity, uint64_t seed, uint64_t sweep,
-                    double delta, int n_hit, double beta_nc, double w_t, double w_s, int aniso,
-                    double *partials, hipStream_t st) {
+static void sweep_g(void *U, const Geo &g, int mu, int parity, const int *list, int nlist,
+                    uint64_t seed, uint64_t sweep, double delta, int n_hit, double beta_nc,
+                    double w_t, double w_s, int aniso, double *partials, hipStream_t st) {
   using link = typename G::link;
-  const dim3 grid(sweep_blocks(g));
+  const dim3 grid(sweep_blocks(g, list ? nlist : -1));
+  if (grid.x == 0) return;
   switch (g.nd) {
     case 2:
-      hipLaunchKernelGGL((k_sweep<G, 2>), grid, dim3(NTS), 0, st, (link *)U, g, mu, parity, seed,
-                         sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials);
+      hipLaunchKernelGGL((k_sweep<G, 2>), grid, dim3(NTS), 0, st, (link *)U, g, mu, parity, list,
+                         nlist, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials);
       break;
     case 3:
-      hipLaunchKernelGGL((k_sweep<G, 3>), grid, dim3(NTS), 0, st, (link *)U, g, mu, parity, seed,
-                         sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials);
+      hipLaunchKernelGGL((k_sweep<G, 3>), grid, dim3(NTS), 0, st, (link *)U, g, mu, parity, list,
+                         nlist, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials);
       break;
     default:
-      hipLaunchKernelGGL((k_sweep<G, 4>), grid, dim3(NTS), 0, st, (link *)U, g, mu, parity, seed,
-                         sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials);
+      hipLaunchKernelGGL((k_sweep<G, 4>), grid, dim3(NTS), 0, st, (link *)U, g, mu, parity, list,
+                         nlist, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials);
       break;
   }
 }
 
-void launch_sweep(int group, int prec, void *U, const Geo &g, int mu, int parity, uint64_t seed,
-                  uint64_t sweep, double delta, int n_hit, double beta_nc, double w_t, double w_s,
-                  int aniso, double *partials, hipStream_t st) {
+void launch_sweep(int group, int prec, void *U, const Geo &g, int mu, int parity, const int *list,
+                  int nlist, uint64_t seed, uint64_t sweep, double delta, int n_hit,
+                  double beta_nc, double w_t, double w_s, int aniso, double *partials,
+                  hipStream_t st) {
+#define SWEEP_ARGS U, g, mu, parity, list, nlist, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso, partials, st
   if (group == 1) {
-    if (prec == 0)
-      sweep_g<GSU2<double>>(U, g, mu, parity, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso,
-                            partials, st);
-    else
-      sweep_g<GSU2<float>>(U, g, mu, parity, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso,
-                           partials, st);
+    if (prec == 0) sweep_g<GSU2<double>>(SWEEP_ARGS);
+    else sweep_g<GSU2<float>>(SWEEP_ARGS);
   } else {
-    if (prec == 0)
-      sweep_g<GU1<double>>(U, g, mu, parity, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso,
-                           partials, st);
-    else
-      sweep_g<GU1<float>>(U, g, mu, parity, seed, sweep, delta, n_hit, beta_nc, w_t, w_s, aniso,
-                          partials, st);
+    if (prec == 0) sweep_g<GU1<double>>(SWEEP_ARGS);
+    else sweep_g<GU1<float>>(SWEEP_ARGS);
   }
+#undef SWEEP_ARGS
 }
 
 }  // namespace su2k

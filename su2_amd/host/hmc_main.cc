@@ -19,8 +19,8 @@
 //   usage: su2_hmc_gpu -f input.yaml [-g su2|u1]
 //
 // An input with a `metropolis:` section runs the Metropolis program instead
-// (metropolis.hpp; keys n_meas, N_hit, delta, seed, heat, N_save, conf_dir,
-// conf_basename, lenghty_conf_name, beta_str_width).
+// (metropolis.hpp; keys n_meas, N_hit, delta, seed, heat, restart, N_save,
+// conf_dir, conf_basename, lenghty_conf_name, beta_str_width).
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -91,6 +91,28 @@ std::string gets(const Map &m, const std::string &k, const std::string &def) {
   return it == m.end() ? def : it->second;
 }
 
+// io::read_nconf_counter (io.hh:267-287): second line "heat i path_conf"
+void read_nconf_counter(const std::string &conf_dir, bool &heat, size_t &i, std::string &path) {
+  std::ifstream nc(conf_dir + "nconf_counter.txt");
+  if (!nc) {
+    std::cerr << conf_dir + "nconf_counter.txt" << ": no such file or directory. Aborting.\n";
+    std::abort();
+  }
+  std::string line;
+  std::getline(nc, line);
+  std::getline(nc, line);
+  std::istringstream ls(line);
+  double h = 0.0;
+  ls >> h >> i >> path;
+  heat = h != 0.0;
+}
+
+// io::update_nconf_counter (io.hh:289-304)
+void write_nconf_counter(const std::string &conf_dir, bool heat, size_t i, const std::string &path) {
+  std::ofstream nc(conf_dir + "nconf_counter.txt", std::ios::out);
+  nc << "heat i path_conf\n" << double(heat) << " " << i << " " << path;
+}
+
 template <class G> int run(const Map &in) {
   namespace S = su2_mi355x;
   size_t Lx, Ly, Lz;
@@ -148,18 +170,8 @@ template <class G> int run(const Map &in) {
   size_t icounter = 0;
   bool g_heat = heat;
   if (restart) {
-    // io::read_nconf_counter (io.hh:267-287): "heat i path_conf"
-    std::ifstream nc(conf_dir + "nconf_counter.txt");
-    if (!nc) {
-      std::cerr << conf_dir + "nconf_counter.txt" << ": no such file or directory. Aborting.\n";
-      std::abort();
-    }
-    std::string line;
-    std::getline(nc, line);
-    std::getline(nc, line);
-    std::istringstream ls(line);
     std::string path;
-    ls >> g_heat >> icounter >> path;
+    read_nconf_counter(conf_dir, g_heat, icounter, path);
     if (U.load(path) != 0) {
       std::cout << "Error: failed to load initial gauge configuration for intializing the "
                    "Markov chain Monte Carlo. Aborting.\n";
@@ -225,8 +237,7 @@ template <class G> int run(const Map &in) {
         gp << res_dir << "/gradient_flow." << std::setw(6) << std::setfill('0') << i;
         S::flat_spacetime::gradient_flow(U, gp.str(), gf_tmax, gf_eps, xi, 0.0, false);
       }
-      std::ofstream nc(conf_dir + "nconf_counter.txt", std::ios::out);
-      nc << "heat i path_conf\n" << double(g_heat) << " " << i << " " << path_i;
+      write_nconf_counter(conf_dir, g_heat, i, path_i);
     }
   }
   std::cout << "## Acceptance rate: " << rate / static_cast<double>(n_meas) << std::endl;
@@ -263,10 +274,7 @@ template <class G> int run_metropolis(const Map &in) {
   const size_t N_hit = get<size_t>(in, "metropolis.N_hit", 10);
   const double delta = get<double>(in, "metropolis.delta", 1.0);
   const bool heat = getb(in, "metropolis.heat", true);
-  if (getb(in, "metropolis.restart", false)) {
-    std::cerr << "Error: metropolis.restart is not supported by this program\n";
-    return 1;
-  }
+  const bool restart = getb(in, "metropolis.restart", false);
   std::string conf_dir = gets(in, "metropolis.conf_dir", ".");
   const std::string basename = gets(in, "metropolis.conf_basename", "conf");
   const bool lengthy = getb(in, "metropolis.lenghty_conf_name", true);
@@ -285,14 +293,32 @@ template <class G> int run_metropolis(const Map &in) {
   const std::string conf_path_basename = conf_dir + "/" + ss.str();
 
   S::gaugeconfig<G> U(Lx, Ly, Lz, Lt, nd, beta, xi, prec);
-  S::hotstart(U, (int)seed, heat);
-  S::random_gauge_trafo(U, 654321);
-  std::ofstream os(conf_dir + "/output.u1-metropolis.data", std::ios::out);
-  os << "## i P E Q P_ss E_ss Q_ss\n";
+  // init_gauge_conf_mcmc (base_program.hpp:276-318): restart from
+  // nconf_counter.txt or hot / cold start, then the gauge transformation
+  size_t icounter = 0;
+  bool g_heat = heat;
+  if (restart) {
+    std::cout << "## restart " << restart << "\n";
+    std::string path;
+    read_nconf_counter(conf_dir, g_heat, icounter, path);
+    if (U.load(path) != 0) {
+      std::cout << "Error: failed to load initial gauge configuration for intializing the "
+                   "Markov chain Monte Carlo. Aborting.\n";
+      std::abort();
+    }
+  } else {
+    S::hotstart(U, (int)seed, g_heat);
+  }
   const double normalisation = 2.0 / nd / (nd - 1) / U.getVolume() / double(U.getNc());
+  std::cout << "## Initial Plaquette: " << S::flat_spacetime::gauge_energy(U) * normalisation
+            << std::endl;
+  S::random_gauge_trafo(U, 654321);
+  std::ofstream os(conf_dir + "/output.u1-metropolis.data", icounter == 0 ? std::ios::out : std::ios::app);
+  os << "## i P E Q P_ss E_ss Q_ss\n";
   const double facnorm = (nd > 2) ? double(nd) / double(nd - 2) : 0.0;
   double rate[2] = {0.0, 0.0};
-  for (size_t i = 0; i < n_meas; ++i) {
+  // run (metropolis.hpp:161-186) with one thread: sweep i = icounter .. icounter + n_meas
+  for (size_t i = icounter; i < n_meas + icounter; ++i) {
     std::vector<std::mt19937> engines(1);
     engines[0].seed(seed + i);
     const std::vector<double> r = S::flat_spacetime::sweep(U, engines, delta, N_hit, beta, xi, aniso);
@@ -310,7 +336,12 @@ template <class G> int run_metropolis(const Map &in) {
     line << "\n";
     std::cout << line.str() << std::flush;
     os << line.str() << std::flush;
-    if (i > 0 && (i % N_save) == 0) U.save(conf_path_basename + "." + std::to_string(i));
+    // do_sweep / after_MCMC_step (metropolis.hpp:124-128, base_program.hpp:439-456)
+    if (i > 0 && (i % N_save) == 0) {
+      const std::string path_i = conf_path_basename + "." + std::to_string(i);
+      U.save(path_i);
+      write_nconf_counter(conf_dir, g_heat, i, path_i);
+    }
   }
   std::cout << "## Acceptance rate " << rate[0] / double(n_meas) << " temporal acceptance rate "
             << rate[1] / double(n_meas) << std::endl;

@@ -132,3 +132,50 @@ def test_mixed_ensemble_runs(S):
         assert len(res) == 224
         assert all(np.isfinite(r["deltaH"]) for r in res)
         assert b.link_updates() > 1e8
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+def test_full_ensemble_member_parity(S, oracle, prec):
+    """The 224-lattice mixed ensemble (128 U(1) d=2, 64 SU(2) d=3, 32 SU(2) d=4,
+    leapfrog with 10..40 steps) after autotuning and with graph replay: the
+    smallest, largest and a middle-sized member of every family follow the
+    trajectory of a single-lattice handle from the same links and host momenta
+    (md_update.hh:18-81): FP64 within 1e-10 per link (and the oracle's
+    trajectory for the SU(2) d=4 members), FP32 within 2e-5."""
+    members = S.ensemble_members()
+    fams = {(0, 2): [], (1, 3): [], (1, 4): []}
+    for i, m in enumerate(members):
+        fams[(m["group"], m["dims"][4])].append(i)
+    pick = []
+    for idx in fams.values():
+        by_size = sorted(idx, key=lambda i: (members[i]["dims"][0], i))
+        pick += [by_size[0], by_size[len(by_size) // 2], by_size[-1]]
+    starts = {}
+    with S.Batch(members, precision=prec) as b:
+        b.autotune(None)
+        b.use_graph(True)
+        for i, m in enumerate(members):
+            P = oracle.initnormal(m["group"], m["dims"], SEED + 500 + i)
+            b.set_momenta(i, P)
+            if i in pick:
+                U = oracle.hotstart(m["group"], m["dims"], SEED + 400 + i, 0.6)
+                b.set_links(i, U)
+                starts[i] = (U, P)
+        res = b.trajectory(0, momenta=S.MOMENTA_HOST, metropolis=False)
+        got = {i: b.get_links(i) for i in pick}
+    for i in pick:
+        m = members[i]
+        U, P = starts[i]
+        integ = S.INTEGRATORS[m["integrator"]] if isinstance(m["integrator"], str) else m["integrator"]
+        with S.Lattice(m["group"], m["dims"], m["beta"], precision=prec) as lat:
+            lat.set_links(U)
+            r = lat.md_trajectory(P, integ, m["nsteps"])
+            lat.md_finish(True)
+            want = lat.get_links()
+        tol = 1e-10 if prec == 0 else 2e-5
+        np.testing.assert_allclose(got[i], want, atol=tol, err_msg=f"member {i} {m['dims']}")
+        assert res[i]["deltaH"] == pytest.approx(r["deltaH"], rel=1e-9 if prec == 0 else 1e-3,
+                                                 abs=1e-9 if prec == 0 else 5e-3)
+        if prec == 0 and m["dims"][4] == 4:
+            Uw, _ = oracle.integrate(m["group"], m["dims"], U, P, m["beta"], integ, m["nsteps"])
+            np.testing.assert_allclose(got[i], Uw, atol=1e-10)

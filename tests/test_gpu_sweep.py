@@ -56,11 +56,37 @@ def test_sweep_fp32_tracks_oracle(S, oracle):
     assert np.abs((q ** 2).sum(1) - 1.0).max() < 5e-7
 
 
-def test_sweep_needs_even_extents(S):
-    with S.Lattice(S.SU2, (4, 4, 4, 3, 4), 2.2) as lat:
-        lat.coldstart()
-        with pytest.raises(S.SU2GPUError, match="even"):
-            lat.sweep(1, 0, 0.4, 1)
+# odd extents: three colour classes per direction over host-built site lists
+@pytest.mark.parametrize("grp,dims,xi", [(1, (4, 4, 4, 3, 4), 1.0), (1, (3, 4, 5, 6, 4), 1.0),
+                                         (1, (5, 5, 5, 5, 4), 1.3), (1, (4, 3, 1, 5, 3), 1.0),
+                                         (0, (5, 1, 1, 7, 2), 1.0), (0, (3, 4, 2, 4, 4), 1.0)])
+def test_sweep_odd_extents_parity_with_oracle(S, oracle, grp, dims, xi):
+    U = oracle.hotstart(grp, dims, SEED + 3, 1.0)
+    with S.Lattice(grp, dims, 2.2, xi=xi) as lat:
+        lat.set_links(U)
+        Uw = U
+        for w in range(2):
+            r = lat.sweep(SEED + 4, w, 0.4, 5)
+            Uw, rw = oracle.sweep(grp, dims, Uw, 2.2, SEED + 4, w, 0.4, 5, xi=xi)
+            assert r == pytest.approx(rw, abs=1e-12)
+        np.testing.assert_allclose(lat.get_links(), Uw, atol=1e-12)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4, 4), (3, 4, 5, 6, 4)])
+def test_sweep_fp32_deterministic_bound(S, oracle, dims):
+    """At beta = 0 every hit is accepted in both forms (dS = 0), so no accept
+    decision can differ and every FP32 link must stay within a rounding bound of
+    the oracle's FP64 link: five hits, each one FP64 product and restoreSU
+    rounded once to FP32 (|error| <= 5 * 2^-24 per component, plus the FP32
+    start), bound 1e-6."""
+    U = oracle.hotstart(1, dims, SEED + 5, 1.0)
+    with S.Lattice(S.SU2, dims, 0.0, precision=S.FP32) as lat:
+        lat.set_links(U)
+        r = lat.sweep(SEED + 6, 0, 0.4, 5)
+        Ug = lat.get_links()
+    Uw, rw = oracle.sweep(1, dims, U, 0.0, SEED + 6, 0, 0.4, 5)
+    assert r[0] == 1.0 and rw[0] == 1.0
+    assert np.abs(Ug - Uw).max() < 1e-6
 
 
 def test_host_program_metropolis(S, oracle, tmp_path):
@@ -108,3 +134,52 @@ monomials:
             E, Q = lat.energy_density(clover=False)
             assert float(row[2]) == pytest.approx(E, rel=1e-13)
     assert "## Acceptance rate" in out.stdout
+
+
+def test_host_program_metropolis_restart(S, tmp_path):
+    """Metropolis restart (init_gauge_conf_mcmc, base_program.hpp:276-318, and
+    after_MCMC_step, :439-456): a run of 5 sweeps saving every 2 leaves
+    nconf_counter.txt at sweep 4; a restart loads conf.4 (its plaquette is the
+    saved configuration's), applies the gauge transformation and continues at
+    sweep 4."""
+    import os
+    import subprocess
+    from su2_amd import host
+    exe = host.EXE
+    assert os.path.exists(exe), "host program not built"
+    conf = tmp_path / "confs"
+    inp = tmp_path / "in.yaml"
+    text = f"""geometry:
+  L : 4
+  T : 5
+  ndims : 4
+metropolis:
+  seed: 77
+  n_meas: 5
+  N_hit: 3
+  N_save: 2
+  delta: 0.4
+  heat: true
+  lenghty_conf_name: false
+  conf_dir: "{conf}/"
+monomials:
+  gauge:
+    beta: 2.2
+"""
+    inp.write_text(text)
+    out = subprocess.run([exe, "-f", str(inp)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [ln.split() for ln in out.stdout.splitlines() if ln and ln[0].isdigit()]
+    assert [int(r[0]) for r in rows] == [0, 1, 2, 3, 4]
+    counter = (conf / "nconf_counter.txt").read_text().splitlines()
+    assert counter[0] == "heat i path_conf" and counter[1].split()[1] == "4"
+    assert (conf / "conf.4").exists()
+    inp.write_text(text.replace("heat: true", "restart: true").replace("n_meas: 5", "n_meas: 3"))
+    out2 = subprocess.run([exe, "-f", str(inp)], capture_output=True, text=True, timeout=120)
+    assert out2.returncode == 0, out2.stderr
+    rows2 = [ln.split() for ln in out2.stdout.splitlines() if ln and ln[0].isdigit()]
+    assert [int(r[0]) for r in rows2] == [4, 5, 6]
+    # the loaded configuration is conf.4, i.e. the chain's state after sweep 4
+    plaq = [float(ln.split()[-1]) for ln in out2.stdout.splitlines() if ln.startswith("## Initial Plaquette")]
+    # printed with the default six significant digits, as the reference does
+    assert plaq and plaq[0] == pytest.approx(float(rows[4][1]), rel=1e-5)

@@ -471,3 +471,38 @@ def test_haarstart_u1_uniform_phase(oracle):
     u = oracle.haarstart(U1, (64, 1, 1, 64, 2), SEED)
     assert u.min() >= -math.pi and u.max() < math.pi
     assert stats.kstest(u, "uniform", args=(-math.pi, 2 * math.pi)).pvalue > 1e-4
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 3, 4), (3, 4, 5, 6, 4), (5, 5, 5, 5, 4), (4, 3, 1, 5, 3),
+                                  (5, 1, 1, 7, 2), (2, 3, 2, 2, 4), (7, 1, 1, 1, 2)])
+def test_sweep_colour_classes_are_independent_sets(oracle, dims):
+    """The sweep's colour classes (orc_sweep_colour): for the links of direction
+    mu, sites that differ by one step along any nu != mu never share a class,
+    so every class is a set of links none of which is in another's staple; and
+    with all extents even the classes are the site parities."""
+    import ctypes as C
+    lib = oracle.lib
+    lib.orc_sweep_colour.restype = C.c_int
+    lib.orc_sweep_colour.argtypes = [np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                     np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), C.c_int]
+    Lx, Ly, Lz, Lt, nd = dims
+    L = [Lt, Lx, Ly, Lz]
+    d = np.asarray(dims, dtype=np.int32)
+    import itertools
+    cols = {}
+    for c in itertools.product(*[range(n) for n in L]):
+        for mu in range(nd):
+            cols[(c, mu)] = lib.orc_sweep_colour(d, np.asarray(c, dtype=np.int32), mu)
+    ncol = max(cols.values()) + 1
+    assert ncol <= 3
+    for (c, mu), k in cols.items():
+        for nu in range(nd):
+            if nu == mu or L[nu] == 1:
+                continue
+            for step in (1, -1):
+                n = list(c)
+                n[nu] = (n[nu] + step) % L[nu]
+                assert cols[(tuple(n), mu)] != k, (c, mu, nu)
+    if all(L[k] % 2 == 0 for k in range(nd)):
+        for (c, mu), k in cols.items():
+            assert k == sum(c) % 2
