@@ -122,6 +122,15 @@ int su2gpu_restoreSU(su2gpu_lattice *h);
 /* retr_sum_Wplaquettes(U, xi, anisotropic, spatial_only) (flat-gauge_energy.hpp:31-94) */
 int su2gpu_retr_sum_plaquettes(su2gpu_lattice *h, double xi, int anisotropic, int spatial_only,
                                double *res);
+/* No reference counterpart; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops).  The sum of su2gpu_retr_sum_plaquettes,
+ * written to device memory (*sum_dev, one double) on the handle's stream,
+ * without synchronising and without allocating: the block partials live in a
+ * static device array of 2 MiB per GPU, shared by every call on that GPU, so
+ * calls that overlap on one GPU race.  Lattices of up to 67,108,352 sites;
+ * larger ones are rejected with SU2GPU_EINVAL. */
+int su2gpu_retr_sum_plaquettes_noalloc(su2gpu_lattice *h, double xi, int anisotropic,
+                                       int spatial_only, double *sum_dev);
 /* gauge_energy(U, spatial_only) (flat-gauge_energy.hpp:109-137) */
 int su2gpu_gauge_energy(su2gpu_lattice *h, int spatial_only, double *res);
 /* get_S_G_hmc (flat-gaugemonomial.hh:46-54) = gaugemonomial heatbath/accept value */

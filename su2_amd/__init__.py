@@ -67,6 +67,7 @@ _SIGS = {
     "su2gpu_haarstart": (C.c_int, [_h, C.c_uint64]),
     "su2gpu_restoreSU": (C.c_int, [_h]),
     "su2gpu_retr_sum_plaquettes": (C.c_int, [_h, C.c_double, C.c_int, C.c_int, _dp]),
+    "su2gpu_retr_sum_plaquettes_noalloc": (C.c_int, [_h, C.c_double, C.c_int, C.c_int, _vp]),
     "su2gpu_gauge_energy": (C.c_int, [_h, C.c_int, _dp]),
     "su2gpu_gauge_action": (C.c_int, [_h, _dp]),
     "su2gpu_kinetic_energy": (C.c_int, [_h, _dp]),
@@ -258,6 +259,14 @@ class Lattice:
         _ck(self.lib.su2gpu_retr_sum_plaquettes(self._h, xi, int(anisotropic), int(spatial_only),
                                                 C.byref(r)))
         return r.value
+
+    def retr_sum_plaquettes_noalloc(self, sum_dev_ptr, xi=1.0, anisotropic=False,
+                                    spatial_only=False):
+        """The plaquette sum written to device memory at sum_dev_ptr (one
+        double) on the handle's stream, with no allocation; the caller
+        synchronises (synchronize()) before reading it."""
+        _ck(self.lib.su2gpu_retr_sum_plaquettes_noalloc(self._h, xi, int(anisotropic),
+                                                        int(spatial_only), C.c_void_p(sum_dev_ptr)))
 
     def gauge_energy(self, spatial_only=False):
         r = C.c_double()

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libsu2mi355x.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["su2_kernels.hip", "su2_flow.hip", "su2_batch.hip", "su2_sweep.hip", "su2_capi.cpp",
-           "su2_batch.cpp"]
+           "su2_batch.cpp", "su2_noalloc.hip"]
 HEADERS = ["su2_device.hpp", "su2_launch.hpp"]
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in any device code
