@@ -480,6 +480,13 @@ static int half_pref() {
   if (v < 0) v = env_int("SU2GPU_HALF", 1);
   return v;
 }
+// SU2GPU_ZG = 0 keeps the FP32 specialised step without lane exchange where
+// Lz does not divide 64 (the general exchange is on by default there)
+static int zg_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_ZG", 1);
+  return v;
+}
 // SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
 static int zs_pref() {
   static int v = -1;
@@ -506,10 +513,10 @@ static void launch_half(bool zs, bool nt, UI uin, QHalfW uout, const FusedLaunch
 
 // the conditions under which launch_fast takes the specialised thread-per-link
 // step for a launch (shared by launch_fast and fused_half_ok)
-static bool fast_geo_for(const Geo &g, FastGeo &f) {
+static bool fast_geo_for(const Geo &g, FastGeo &f, bool wmu = false) {
   const int xw = xwalk_pref();
   const bool xwalk = xw >= 0 ? xw != 0 : (g.L[2] * g.L[3]) / NTF >= 16;
-  return make_fast_geo(g.L, g.V, tblock_pref() != 0, f, xwalk);
+  return make_fast_geo(g.L, g.V, tblock_pref() != 0, f, xwalk, wmu);
 }
 
 bool fused_half_ok(int group, int prec, const Geo &g, int flags) {
@@ -571,6 +578,35 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     const int ntp = nt_pref();
     const double fields = 4.0 * L.geo.V * (2.0 * sizeof(quat<T>) + 3.0 * sizeof(T));
     const bool nt = ntp >= 0 ? ntp != 0 : fields > 256.0 * 1024 * 1024;
+    // FP32 lattices whose Lz does not divide 64: the general +-z lane exchange
+    // (neighbours in another wave loaded by the lanes that need them), the
+    // four directions of 64 sites per block, and beyond the Infinity Cache the
+    // momenta loaded non-temporally (48^4: 466-483 -> 411 us; FP64 keeps the
+    // loads, where the exchange's registers cost a wave per SIMD)
+    FastGeo fw;
+    if constexpr (sizeof(T) == 4) {
+    if (!site && !an && !zs && zs_pref() && zg_pref() && !L.in_half && !L.out_half &&
+        fast_geo_for(L.geo, fw, true)) {
+      const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
+#define SU2_ZG(W, A, N)                                                                         \
+  hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, 2, false, N>), grid, block, 0, st,               \
+                     (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, fw, L.c, L.partials, \
+                     L.pstride)
+#define SU2_ZG2(W, A) \
+  if (nt) SU2_ZG(W, A, 2); \
+  else SU2_ZG(W, A, 0)
+      if (L.Uout) {
+        if (acc) { SU2_ZG2(true, true); }
+        else { SU2_ZG2(true, false); }
+      } else {
+        if (acc) { SU2_ZG2(false, true); }
+        else { SU2_ZG2(false, false); }
+      }
+#undef SU2_ZG2
+#undef SU2_ZG
+      return true;
+    }
+    }
     if (L.in_half || L.out_half) {
       // the host asked for it only where fused_half_ok holds
       if constexpr (sizeof(T) == 8) {

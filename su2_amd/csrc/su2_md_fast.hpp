@@ -70,6 +70,12 @@ struct FastGeo {
   int xwalk;      // tblock walk: 3-volume blocks x-fastest (bx blocks per x plane)
   int bx;         // plane / NTF
   uint32_t m_plane, m_Lz, m_Tc, m_nb3, m_Lx;  // multiply-high divisors (fast_div)
+  // wmu: a block holds 64 sites and its four waves the four directions mu of
+  // them (wave w: mu = w), so the links the directions share are requested by
+  // waves of one CU at about the same time
+  int wmu;
+  int nb64;       // vol3 / 64
+  uint32_t m_nb64;
 };
 
 // floor(n / d) = umulhi(n, m) with m = floor(2^32 / d) + 1, exact whenever
@@ -97,6 +103,27 @@ __host__ __device__ __forceinline__ FastSite fast_site(const FastGeo &g, uint32_
                                                        uint32_t tid) {
   FastSite f;
   int t, blk3;
+  uint32_t o3;
+  if (g.wmu) {
+    // block: 64 sites, wave w: direction w; XCD k walks (64-site block, t)
+    // with t fastest inside its t-chunk (tblock), or one eighth of the
+    // (t, block) sequence
+    f.mu = (int)(tid >> 6);
+    const uint32_t xcd = b & 7u, local = b >> 3;
+    int bl;
+    if (g.tblock) {
+      const uint32_t q = fast_div(local, g.m_Tc);
+      bl = (int)q;
+      t = (int)xcd * g.Tc + (int)(local - q * (uint32_t)g.Tc);
+    } else {
+      const uint32_t per = nb >> 3, rem = nb & 7u;
+      const uint32_t lb = xcd * per + (xcd < rem ? xcd : rem) + local;
+      const uint32_t q = fast_div(lb, g.m_nb64);
+      t = (int)q;
+      bl = (int)(lb - q * (uint32_t)g.nb64);
+    }
+    o3 = (uint32_t)bl * 64u + (tid & 63u);
+  } else {
   if (g.tblock) {
     const uint32_t xcd = b & 7u, local = b >> 3;
     f.mu = (int)(local & 3u);
@@ -119,7 +146,8 @@ __host__ __device__ __forceinline__ FastSite fast_site(const FastGeo &g, uint32_
     t = (int)q;
     blk3 = (int)(sb - q * (uint32_t)g.nb3);
   }
-  const uint32_t o3 = (uint32_t)blk3 * NTF + tid;
+  o3 = (uint32_t)blk3 * NTF + tid;
+  }
   const uint32_t x = fast_div(o3, g.m_plane);
   const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
   const uint32_t y = fast_div(rem3, g.m_Lz);
@@ -141,7 +169,7 @@ __host__ __device__ __forceinline__ FastSite fast_site(const FastGeo &g, uint32_
 // 32-bit byte offsets within one direction plane, and every multiply-high
 // division within its exact range (n d < 2^32)
 inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f,
-                          bool allow_xwalk = false) {
+                          bool allow_xwalk = false, bool wmu = false) {
   f.Lt = L[0];
   f.Lx = L[1];
   f.Ly = L[2];
@@ -165,6 +193,12 @@ inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f,
   f.xwalk = (allow_xwalk && f.tblock && f.plane % NTF == 0) ? 1 : 0;
   f.bx = f.plane / NTF;
   f.m_Lx = fast_div_magic(f.Lx);
+  f.wmu = wmu ? 1 : 0;
+  f.nb64 = f.vol3 / 64;
+  // block counts divided: local < V / 64 / 8 by Tc, lb < V / 64 by nb64
+  if (wmu && ((uint64_t)V / 64 * f.nb64 >= two32 || (uint64_t)V / 64 * f.Tc >= two32)) return false;
+  f.m_nb64 = fast_div_magic(f.nb64);
+  if (wmu) f.xwalk = 0;
   return true;
 }
 
@@ -192,9 +226,14 @@ __device__ __forceinline__ void sto_nt(quat<double> *__restrict__ base, uint32_t
   __builtin_nontemporal_store(a, (su2_d2 *)((char *)base + i * 32u));
   __builtin_nontemporal_store(b, (su2_d2 *)((char *)base + i * 32u + 16u));
 }
-template <bool NT, typename E> __device__ __forceinline__ void sto_h(E *__restrict__ base, uint32_t i, E v) {
-  if constexpr (NT) sto_nt(base, i, v);
+template <int NT, typename E> __device__ __forceinline__ void sto_h(E *__restrict__ base, uint32_t i, E v) {
+  if constexpr (NT != 0) sto_nt(base, i, v);
   else sto(base, i, v);
+}
+// a momentum component, with the non-temporal hint (NTL) or without
+template <bool NTL, typename E> __device__ __forceinline__ E ldm(const E *__restrict__ base, uint32_t i) {
+  if constexpr (NTL) return __builtin_nontemporal_load((const E *)((const char *)base + i * (uint32_t)sizeof(E)));
+  else return ldo(base, i);
 }
 
 // Half-field layout of an FP64 SU(2) link field of n links: the 16-B halves
@@ -226,14 +265,14 @@ __device__ __forceinline__ quat<double> ldq(QHalf f, uint32_t i) {
   const su2_d2 x = ldo(f.a, i), y = ldo(f.b, i);
   return quat<double>{x.x, x.y, y.x, y.y};
 }
-template <bool NT, typename T>
+template <int NT, typename T>
 __device__ __forceinline__ void stq(quat<T> *__restrict__ p, uint32_t i, const quat<T> &v) {
   sto_h<NT>(p, i, v);
 }
-template <bool NT>
+template <int NT>
 __device__ __forceinline__ void stq(QHalfW f, uint32_t i, const quat<double> &v) {
   const su2_d2 x = {v.ar, v.ai}, y = {v.br, v.bi};
-  if constexpr (NT) {
+  if constexpr (NT != 0) {
     __builtin_nontemporal_store(x, (su2_d2 *)((char *)f.a + i * 16u));
     __builtin_nontemporal_store(y, (su2_d2 *)((char *)f.b + i * 16u));
   } else {
@@ -276,7 +315,7 @@ __device__ __forceinline__ void acc_dw(double (&K)[4], double w, const quat<T> &
 // Momentum and link update of one link from its staple sum K (FP64):
 // M = U K, P <- P - ep coef 2 Im(M) rounded once, U' = exp(eu P) U.
 // Returns Re(M) and the new momentum for the block sums.
-template <typename T, bool WRITE_U, bool NT = false, class UO = quat<T> *>
+template <typename T, bool WRITE_U, int NT = 0, class UO = quat<T> *>
 __device__ __forceinline__ double md_su2_update(const quat<T> &u, const double (&K)[4], T pi0,
                                                 T pi1, T pi2, T *__restrict__ P0,
                                                 T *__restrict__ P1, T *__restrict__ P2,
@@ -366,8 +405,10 @@ __device__ __forceinline__ void staple_pair(double (&K)[4], double w, const quat
 // dimensions (direction ND - 1 the fastest); returns the input momentum (pi),
 // the new one (pn) and Re(U K) (mar) for the sums.  Of the geometry it reads
 // g.V and, with ZS, g.Lz (the extent of direction ND - 1).
-template <typename T, int MU, bool WRITE_U, bool ZS = false, bool AN = false, int ND = 4,
-          bool NT = false, bool POUT = false, class UI = const quat<T> *, class UO = quat<T> *>
+// NT: 0 ordinary stores, 1 non-temporal stores of the new links and
+// momenta, 2 also non-temporal loads of the momenta
+template <typename T, int MU, bool WRITE_U, int ZS = 0, bool AN = false, int ND = 4,
+          int NT = 0, bool POUT = false, class UI = const quat<T> *, class UO = quat<T> *>
 __device__ __forceinline__ void md_su2_core(UI Uin, UO Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
                                             const FastSite &fs, T (&pi)[3], T (&pn)[3],
@@ -381,21 +422,38 @@ __device__ __forceinline__ void md_su2_core(UI Uin, UO Uout, T *__restrict__ P,
   T *__restrict__ P1 = P + (size_t)(mu * 3 + 1) * V;
   T *__restrict__ P2 = P + (size_t)(mu * 3 + 2) * V;
   const quat<T> u = ldq(Um, (uint32_t)s);
-  const T pi0 = ldo(P0, (uint32_t)s), pi1 = ldo(P1, (uint32_t)s), pi2 = ldo(P2, (uint32_t)s);
+  // NT == 2: the momenta, read once per step, also load with the
+  // non-temporal hint (they do not displace the links from the caches)
+  const T pi0 = ldm<NT == 2>(P0, (uint32_t)s), pi1 = ldm<NT == 2>(P1, (uint32_t)s),
+          pi2 = ldm<NT == 2>(P2, (uint32_t)s);
   // -0.0 is the identity of FP64 addition (+0.0 is not: +0.0 + -0.0 = +0.0),
   // so the first staple's addition folds away
   double K[4] = {-0.0, -0.0, -0.0, -0.0};
   if constexpr (MU >= 0) {
     quat<T> A[ND - 1], B[ND - 1], C[ND - 1], D[ND - 1], E[ND - 1], F[ND - 1];
-    // ZS: the +-z neighbours of the wave's sites sit in the neighbouring lanes
-    // (the wave holds whole z-rows), so they are exchanged through ds_bpermute
-    // instead of loaded: 4 of the 6 loads of the plane nu = z, 2 of 6 in every
-    // plane when mu = z
+    // ZS: the +-z neighbours of the wave's sites sit in the neighbouring lanes,
+    // so they are exchanged through ds_bpermute instead of loaded: 4 of the 6
+    // loads of the plane nu = z, 2 of 6 in every plane when mu = z.
+    //   ZS = 1: the wave holds whole z-rows (64 % Lz == 0), every neighbour is
+    //           in the wave;
+    //   ZS = 2: any Lz; the wave holds 64 consecutive sites, so the neighbour
+    //           s +- z of lane l is lane l + up_z / l + dn_z where that lies in
+    //           [0, 64), and the few lanes whose neighbour is in another wave
+    //           (a row's wrap, the wave's ends) load it.
     int zup = 0, zdn = 0;
-    if (ZS) {
+    bool vup = true, vdn = true;
+    if (ZS == 1) {
       const int lane = threadIdx.x & 63, zl = lane & (g.Lz - 1);
       zup = (zl == g.Lz - 1) ? lane - (g.Lz - 1) : lane + 1;
       zdn = (zl == 0) ? lane + (g.Lz - 1) : lane - 1;
+    } else if (ZS == 2) {
+      const int lane = threadIdx.x & 63;
+      zup = lane + up[ND - 1];
+      zdn = lane + dn[ND - 1];
+      vup = (unsigned)zup < 64u;
+      vdn = (unsigned)zdn < 64u;
+      if (!vup) zup = lane;
+      if (!vdn) zdn = lane;
     }
 #pragma unroll
     for (int k = 0; k < ND - 1; ++k) {
@@ -432,6 +490,27 @@ __device__ __forceinline__ void md_su2_core(UI Uin, UO Uout, T *__restrict__ P,
           D[k] = qshfl(F[k], zup);  // U_nu(s+z-nu)
         }
       }
+      if (ZS == 2) {
+        // neighbours in another wave: loaded by the lanes that need them
+#pragma unroll
+        for (int k = 0; k < ND - 1; ++k) {
+          const int nu = k + (k >= MU ? 1 : 0);
+          const UI Un = fld_add(Uin, (size_t)nu * V);
+          if (nu == ND - 1) {
+            if (!vup) B[k] = ldq(Um, (uint32_t)(s + up[ND - 1]));
+            if (!vdn) {
+              E[k] = ldq(Um, (uint32_t)(s + dn[ND - 1]));
+              D[k] = ldq(Un, (uint32_t)(s + up[MU] + dn[ND - 1]));
+              F[k] = ldq(Un, (uint32_t)(s + dn[ND - 1]));
+            }
+          } else if (MU == ND - 1) {
+            if (!vup) {
+              A[k] = ldq(Un, (uint32_t)(s + up[ND - 1]));
+              D[k] = ldq(Un, (uint32_t)(s + up[ND - 1] + dn[nu]));
+            }
+          }
+        }
+      }
     }
 #pragma unroll
     for (int k = 0; k < ND - 1; ++k) {
@@ -460,8 +539,8 @@ __device__ __forceinline__ void md_su2_core(UI Uin, UO Uout, T *__restrict__ P,
                                           fld_add(Uout, (size_t)mu * V), (uint32_t)s, c, pn);
 }
 
-template <typename T, int MU, bool WRITE_U, bool ACCUM, bool ZS = false, bool AN = false,
-          bool NT = false, bool POUT = false, class UI = const quat<T> *, class UO = quat<T> *>
+template <typename T, int MU, bool WRITE_U, bool ACCUM, int ZS = 0, bool AN = false,
+          int NT = 0, bool POUT = false, class UI = const quat<T> *, class UO = quat<T> *>
 __device__ __forceinline__ void md_su2_body(UI Uin, UO Uout, T *__restrict__ P,
                                             const FastGeo &g, const FusedCoef &c,
                                             const FastSite &fs, double *__restrict__ partials,
@@ -483,39 +562,41 @@ __device__ __forceinline__ void md_su2_body(UI Uin, UO Uout, T *__restrict__ P,
 
 // SPEC: 1 = the block's mu selects one of four bodies with mu fixed at compile
 // time (mu is uniform per block); 0 = one body with mu at run time
-// ZS (with SPEC; needs 64 % Lz == 0): +-z neighbours by lane exchange
+// ZS (with SPEC): +-z neighbours by lane exchange, 1: 64 % Lz == 0, 2: any Lz
 // AN: anisotropic weights w_t / w_s on the planes (FMA accumulation)
 // UI / UO: the link field types read and written (record-layout pointers, or
 // QHalf / QHalfW for the FP64 half-field layout)
-template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, bool ZS = false, bool AN = false,
-          bool NT = false, class UI = const quat<T> *, class UO = quat<T> *>
+template <typename T, bool WRITE_U, bool ACCUM, int SPEC = 1, int ZS = 0, bool AN = false,
+          int NT = 0, class UI = const quat<T> *, class UO = quat<T> *>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d(UI Uin, UO Uout,
                                                  T *__restrict__ P, FastGeo g, FusedCoef c,
                                                  double *__restrict__ partials, int pstride) {
   __shared__ double lds[3 * NTF / 64];
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
   if (SPEC) {
-    switch (fs.mu) {
+    // mu is uniform per wave (per block, or wave index with wmu): read it as
+    // such so that the bodies are selected by a scalar branch
+    switch (__builtin_amdgcn_readfirstlane(fs.mu)) {
       case 0: md_su2_body<T, 0, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
       case 1: md_su2_body<T, 1, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
       case 2: md_su2_body<T, 2, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
       default: md_su2_body<T, 3, WRITE_U, ACCUM, ZS, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds); break;
     }
   } else {
-    md_su2_body<T, -1, WRITE_U, ACCUM, false, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
+    md_su2_body<T, -1, WRITE_U, ACCUM, 0, AN, NT, false, UI, UO>(Uin, Uout, P, g, c, fs, partials, pstride, lds);
   }
 }
 
 // The same step reading the momenta from P and writing the new ones to Pw (a
 // separate buffer of the same layout): the gradient-flow recomputation keeps
 // every stage's Z' without copying it first.
-template <typename T, bool WRITE_U, bool ZS>
+template <typename T, bool WRITE_U, int ZS>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d_pout(const quat<T> *__restrict__ Uin,
                                                       quat<T> *__restrict__ Uout,
                                                       T *__restrict__ P, T *__restrict__ Pw,
                                                       FastGeo g, FusedCoef c) {
   const FastSite fs = fast_site(g, blockIdx.x, gridDim.x, threadIdx.x);
-  switch (fs.mu) {
+  switch (__builtin_amdgcn_readfirstlane(fs.mu)) {
     case 0: md_su2_body<T, 0, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;
     case 1: md_su2_body<T, 1, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;
     case 2: md_su2_body<T, 2, WRITE_U, false, ZS, false, false, true>(Uin, Uout, P, g, c, fs, nullptr, 0, nullptr, Pw); break;

@@ -10,10 +10,10 @@
 
 using namespace su2k;
 
-static int check(const int L[4], bool tblock, bool xwalk) {
+static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
   const int V = L[0] * L[1] * L[2] * L[3];
   FastGeo g;
-  if (!make_fast_geo(L, V, tblock, g, xwalk)) {
+  if (!make_fast_geo(L, V, tblock, g, xwalk, wmu)) {
     printf("%dx%dx%dx%d tblock=%d: not applicable\n", L[0], L[1], L[2], L[3], (int)tblock);
     return 0;
   }
@@ -27,6 +27,10 @@ static int check(const int L[4], bool tblock, bool xwalk) {
   for (uint32_t b = 0; b < nb; ++b) {
     for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
       const FastSite f = fast_site(g, b, nb, tid);
+      if (wmu && f.mu != (int)(tid >> 6)) {
+        printf("FAIL wmu: wave %u of block %u has mu %d\n", tid >> 6, b, f.mu);
+        return 1;
+      }
       if (f.mu < 0 || f.mu > 3 || f.s < 0 || f.s >= V) {
         printf("FAIL out of range b=%u tid=%u mu=%d s=%d\n", b, tid, f.mu, f.s);
         return 1;
@@ -85,8 +89,8 @@ static int check(const int L[4], bool tblock, bool xwalk) {
       }
     }
   }
-  printf("%dx%dx%dx%d tblock=%d xwalk=%d: ok (%u blocks)\n", L[0], L[1], L[2], L[3], g.tblock,
-         g.xwalk, nb);
+  printf("%dx%dx%dx%d tblock=%d xwalk=%d wmu=%d: ok (%u blocks)\n", L[0], L[1], L[2], L[3], g.tblock,
+         g.xwalk, g.wmu, nb);
   return 0;
 }
 
@@ -99,7 +103,7 @@ int main() {
   int fails = 0;
   for (const auto &L : geoms)
     for (int tb = 0; tb < 2; ++tb)
-      for (int xw = 0; xw < 2; ++xw) fails += check(L, tb != 0, xw != 0);
+      for (int xw = 0; xw < 3; ++xw) fails += check(L, tb != 0, xw == 1, xw == 2);
   // Regression: a divisor of 1 has no 32-bit multiply-high constant
   // (floor(2^32 / 1) + 1 = 2^32 + 1 truncates to 1, and umulhi(n, 1) = 0).  With
   // Lt = 8 the t-chunk per XCD is Tc = 1, so that constant would give q = 0 for

@@ -22,11 +22,15 @@ def test_fast_step_layout(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip().endswith("ALL OK")
-    assert "32x32x32x32 tblock=1 xwalk=0: ok" in out.stdout
-    assert "32x32x32x32 tblock=1 xwalk=1: ok" in out.stdout
-    assert "48x48x48x48 tblock=1 xwalk=1: ok" in out.stdout
-    assert "8x8x8x8 tblock=1 xwalk=0: ok" in out.stdout
-    assert "24x24x24x24 tblock=1 xwalk=0: ok" in out.stdout
+    assert "32x32x32x32 tblock=1 xwalk=0 wmu=0: ok" in out.stdout
+    assert "32x32x32x32 tblock=1 xwalk=1 wmu=0: ok" in out.stdout
+    assert "32x32x32x32 tblock=1 xwalk=0 wmu=1: ok" in out.stdout
+    assert "32x32x32x32 tblock=0 xwalk=0 wmu=1: ok" in out.stdout
+    assert "48x48x48x48 tblock=1 xwalk=1 wmu=0: ok" in out.stdout
+    assert "48x48x48x48 tblock=1 xwalk=0 wmu=1: ok" in out.stdout
+    assert "8x8x8x8 tblock=1 xwalk=0 wmu=0: ok" in out.stdout
+    assert "8x8x8x8 tblock=1 xwalk=0 wmu=1: ok" in out.stdout
+    assert "24x24x24x24 tblock=1 xwalk=0 wmu=0: ok" in out.stdout
     # the Lt = 8 layout (Tc = 1) with the truncated divide-by-1 constant goes
     # out of range; the library codes that divisor separately
     assert "8x8x8x8 truncated d=1 multiplier" in out.stdout
