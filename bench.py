@@ -339,7 +339,8 @@ def measured_traffic(L, form):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor that
     the same counter reads for a copy of known size); None if no summary for
-    this lattice size and form is committed."""
+    this lattice size and form is committed, or if it was taken of another
+    instantiation than the one this build launches."""
     path = os.path.join(ROOT, "profiles", "fused_step_traffic.json")
     try:
         with open(path) as f:
@@ -347,7 +348,9 @@ def measured_traffic(L, form):
     except (OSError, ValueError):
         return None
     rec = tab.get(f"{L}^4/{form}")
-    return None if rec is None else rec.get("hbm_bytes_per_launch")
+    if rec is None or rec.get("kernel") != kernel_name(0 if form == "fp64" else 1, L):
+        return None
+    return rec.get("hbm_bytes_per_launch")
 
 
 def progress(msg):
@@ -357,13 +360,22 @@ def progress(msg):
 
 def kernel_name(prec, L):
     """Instantiation of the specialised step that a P-then-U launch of the
-    L^4 SU(2) lattice takes (su2_kernels.hip launch_fast): template flags
-    <T, WRITE_U, ACCUM, SPEC, ZS (64 % L == 0), AN, NT (fields > 256 MB)>."""
-    t = "double" if prec == 0 else "float"
+    L^4 SU(2) lattice takes (su2_kernels.hip launch_fast), as rocprofv3 names
+    it: <T, WRITE_U, ACCUM, SPEC, ZS, AN, NT, link in, link out>.  ZS: 1 the
+    +-z lane exchange within a wave (64 % L == 0), 2 the general exchange
+    (FP32 beyond 256 MiB otherwise), 0 none.  NT: non-temporal stores (1)
+    and momentum loads (2, the FP32 general-exchange step) where the two link
+    fields and the momenta exceed 256 MiB.  The FP64 form steps on the
+    half-field layout (QHalf in, QHalfW out)."""
     r = 8 if prec == 0 else 4
-    zs = "true" if 64 % L == 0 else "false"
-    nt = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
-    return f"k_md_su2_4d<{t},true,false,1,{zs},false" + (",true>" if nt else ">")
+    big = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
+    if prec == 0:
+        zs = 1 if 64 % L == 0 else 0
+        return f"k_md_su2_4d<double, true, false, 1, {zs}, false, {int(big)}, su2k::QHalf, su2k::QHalfW>"
+    zs = 1 if 64 % L == 0 else (2 if big else 0)
+    nt = (2 if zs == 2 else 1) if big else 0
+    return (f"k_md_su2_4d<float, true, false, 1, {zs}, false, {nt}, su2dev::quat<float> const*, "
+            "su2dev::quat<float>*>")
 
 
 def form_roofline(f, prec, L):
@@ -371,7 +383,8 @@ def form_roofline(f, prec, L):
             "frac": f["fused_achieved_GBs"] / HBM_PEAK_GBS, "kernel": kernel_name(prec, L),
             "avg_us": f["fused_avg_us"], "bytes_per_launch": f["fused_bytes_per_launch"],
             "compulsory_frac": f["fused_compulsory_GBs"] / HBM_PEAK_GBS,
-            "trajectory_frac": f["traj_achieved_GBs"] / HBM_PEAK_GBS}
+            "trajectory_frac": f["traj_achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(L, "fp64" if prec == 0 else "fp32")}
 
 
 def main():
@@ -424,8 +437,7 @@ def main():
         for name, prec in (("fp64", S.FP64), ("fp32", S.FP32)):
             progress(f"{name} form at {Lp}^4")
             r = run_form(S, prec, Lp, args.nsteps, 3, 1, args.seed, args.prec_beta)
-            leg48[name] = dict(r, roofline=form_roofline(r, prec, Lp),
-                               traffic=measured_traffic(Lp, name))
+            leg48[name] = dict(r, roofline=form_roofline(r, prec, Lp))
         if not args.no_precision:
             progress(f"FP32 vs FP64 dH at {Lp}^4")
             ddh48 = delta_dH(S, Lp, args.nsteps, args.prec_beta, args.seed, args.ddh_traj,
@@ -454,9 +466,8 @@ def main():
         cpu = cpu_baseline(args.L, args.nsteps, args.beta, args.seed)
     n = args.nsteps
     roof = form_roofline(f64, 0, args.L)
-    roof["traffic"] = measured_traffic(args.L, "fp64")
     roof["traffic_source"] = "profiles/fused_step_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE)"
-    fp32 = dict(f32, roofline=form_roofline(f32, 1, args.L), traffic=measured_traffic(args.L, "fp32"))
+    fp32 = dict(f32, roofline=form_roofline(f32, 1, args.L))
     if ddh32 is not None:
         worst = max(ddh32["max_abs_dH32_minus_dH64"], ddh32["max_abs_ddH_fp32"])
         fp32["delta_dH"] = ddh32

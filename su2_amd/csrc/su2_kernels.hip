@@ -480,11 +480,13 @@ static int half_pref() {
   if (v < 0) v = env_int("SU2GPU_HALF", 1);
   return v;
 }
-// SU2GPU_ZG = 0 keeps the FP32 specialised step without lane exchange where
-// Lz does not divide 64 (the general exchange is on by default there)
+// SU2GPU_ZG = 1 / 0: the general +-z lane exchange in the FP32 specialised
+// step where Lz does not divide 64; default: where the fields exceed the
+// Infinity Cache (48^4: 388-397 against 454-478 us without; 24^4: 24.5 against
+// 23.2 us)
 static int zg_pref() {
-  static int v = -1;
-  if (v < 0) v = env_int("SU2GPU_ZG", 1);
+  static int v = -2;
+  if (v == -2) v = env_int("SU2GPU_ZG", -1);
   return v;
 }
 // SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
@@ -578,15 +580,17 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     const int ntp = nt_pref();
     const double fields = 4.0 * L.geo.V * (2.0 * sizeof(quat<T>) + 3.0 * sizeof(T));
     const bool nt = ntp >= 0 ? ntp != 0 : fields > 256.0 * 1024 * 1024;
-    // FP32 lattices whose Lz does not divide 64: the general +-z lane exchange
-    // (neighbours in another wave loaded by the lanes that need them), the
-    // four directions of 64 sites per block, and beyond the Infinity Cache the
-    // momenta loaded non-temporally (48^4: 466-483 -> 411 us; FP64 keeps the
-    // loads, where the exchange's registers cost a wave per SIMD)
+    // FP32 lattices beyond the Infinity Cache whose Lz does not divide 64: the
+    // general +-z lane exchange (neighbours in another wave loaded by the
+    // lanes that need them, in the first memory round trip), the four
+    // directions of 64 sites per block, and the momenta loaded non-temporally
+    // (48^4: 454-478 -> 388-397 us; FP64 keeps the loads, 850 against 943 us,
+    // where the exchange's registers cost a wave per SIMD)
     FastGeo fw;
+    const int zg = zg_pref();
     if constexpr (sizeof(T) == 4) {
-    if (!site && !an && !zs && zs_pref() && zg_pref() && !L.in_half && !L.out_half &&
-        fast_geo_for(L.geo, fw, true)) {
+    if (!site && !an && !zs && zs_pref() && (zg >= 0 ? zg != 0 : nt) && !L.in_half &&
+        !L.out_half && fast_geo_for(L.geo, fw, true)) {
       const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
 #define SU2_ZG(W, A, N)                                                                         \
   hipLaunchKernelGGL((k_md_su2_4d<T, W, A, 1, 2, false, N>), grid, block, 0, st,               \

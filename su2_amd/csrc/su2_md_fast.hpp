@@ -476,38 +476,52 @@ __device__ __forceinline__ void md_su2_core(UI Uin, UO Uout, T *__restrict__ P,
         F[k] = ldq(Un, (uint32_t)(s + dn[nu]));           // U_nu(s-nu)
       }
     }
+    if (ZS == 2) {
+      // neighbours in another wave: loaded by the lanes that need them, in
+      // the same memory round trip as the other loads (the exchange below
+      // keeps them)
+#pragma unroll
+      for (int k = 0; k < ND - 1; ++k) {
+        const int nu = k + (k >= MU ? 1 : 0);
+        const UI Un = fld_add(Uin, (size_t)nu * V);
+        if (nu == ND - 1) {
+          if (!vup) B[k] = ldq(Um, (uint32_t)(s + up[ND - 1]));
+          if (!vdn) {
+            E[k] = ldq(Um, (uint32_t)(s + dn[ND - 1]));
+            D[k] = ldq(Un, (uint32_t)(s + up[MU] + dn[ND - 1]));
+            F[k] = ldq(Un, (uint32_t)(s + dn[ND - 1]));
+          }
+        } else if (MU == ND - 1) {
+          if (!vup) {
+            A[k] = ldq(Un, (uint32_t)(s + up[ND - 1]));
+            D[k] = ldq(Un, (uint32_t)(s + up[ND - 1] + dn[nu]));
+          }
+        }
+      }
+    }
     if (ZS) {
+      // every lane takes part in each exchange; with ZS = 2 a lane keeps its
+      // loaded value where its neighbour is in another wave
 #pragma unroll
       for (int k = 0; k < ND - 1; ++k) {
         const int nu = k + (k >= MU ? 1 : 0);
         if (nu == ND - 1) {
-          B[k] = qshfl(u, zup);     // U_mu(s+z)
-          E[k] = qshfl(u, zdn);     // U_mu(s-z)
-          D[k] = qshfl(A[k], zdn);  // U_z(s+mu-z)
-          F[k] = qshfl(C[k], zdn);  // U_z(s-z)
+          const quat<T> b = qshfl(u, zup);     // U_mu(s+z)
+          const quat<T> e = qshfl(u, zdn);     // U_mu(s-z)
+          const quat<T> d = qshfl(A[k], zdn);  // U_z(s+mu-z)
+          const quat<T> f = qshfl(C[k], zdn);  // U_z(s-z)
+          if (ZS == 1 || vup) B[k] = b;
+          if (ZS == 1 || vdn) {
+            E[k] = e;
+            D[k] = d;
+            F[k] = f;
+          }
         } else if (MU == ND - 1) {
-          A[k] = qshfl(C[k], zup);  // U_nu(s+z)
-          D[k] = qshfl(F[k], zup);  // U_nu(s+z-nu)
-        }
-      }
-      if (ZS == 2) {
-        // neighbours in another wave: loaded by the lanes that need them
-#pragma unroll
-        for (int k = 0; k < ND - 1; ++k) {
-          const int nu = k + (k >= MU ? 1 : 0);
-          const UI Un = fld_add(Uin, (size_t)nu * V);
-          if (nu == ND - 1) {
-            if (!vup) B[k] = ldq(Um, (uint32_t)(s + up[ND - 1]));
-            if (!vdn) {
-              E[k] = ldq(Um, (uint32_t)(s + dn[ND - 1]));
-              D[k] = ldq(Un, (uint32_t)(s + up[MU] + dn[ND - 1]));
-              F[k] = ldq(Un, (uint32_t)(s + dn[ND - 1]));
-            }
-          } else if (MU == ND - 1) {
-            if (!vup) {
-              A[k] = ldq(Un, (uint32_t)(s + up[ND - 1]));
-              D[k] = ldq(Un, (uint32_t)(s + up[ND - 1] + dn[nu]));
-            }
+          const quat<T> a = qshfl(C[k], zup);  // U_nu(s+z)
+          const quat<T> d = qshfl(F[k], zup);  // U_nu(s+z-nu)
+          if (ZS == 1 || vup) {
+            A[k] = a;
+            D[k] = d;
           }
         }
       }

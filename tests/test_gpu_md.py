@@ -676,7 +676,9 @@ sys.path.insert(0, sys.argv[1])
 import su2_amd as S
 from oracle.oracle import Oracle
 orc = Oracle()
-for dims in [(8, 8, 8, 8, 4), (2, 64, 64, 8, 4)]:
+# Lz = 12 (does not divide 64): the FP32 step's general z exchange (neighbours
+# in another wave loaded by the lanes that need them)
+for dims in [(8, 8, 8, 8, 4), (2, 64, 64, 8, 4), (8, 8, 16, 12, 4)]:
     U = orc.hotstart(1, dims, 77, 0.3)
     P = orc.initnormal(1, dims, 78)
     Uw, _ = orc.integrate(1, dims, U, P, 2.4, 6, 5)
@@ -694,13 +696,16 @@ print("NT_CHILD_OK", flush=True)
 
 
 @pytest.mark.parametrize("env", [{"SU2GPU_NT": "1", "SU2GPU_XWALK": "1"},
-                                 {"SU2GPU_NT": "1", "SU2GPU_XWALK": "0"}],
-                         ids=["nt_xwalk", "nt"])
+                                 {"SU2GPU_NT": "1", "SU2GPU_XWALK": "0"},
+                                 {"SU2GPU_NT": "0", "SU2GPU_ZG": "1"}],
+                         ids=["nt_xwalk", "nt", "zg"])
 def test_nontemporal_store_step_parity(env):
-    """The specialised step with its non-temporal stores forced on (quat<float>
-    and quat<double> sto_nt) on small lattices, in a child process because the
-    switches are read once per process: both forms against the oracle's FP64
-    trajectory from the same links and host momenta."""
+    """The specialised step with its non-temporal stores (quat<float> and
+    quat<double> sto_nt, with the FP32 general z exchange where Lz does not
+    divide 64), or with the general exchange alone, forced on small lattices,
+    in a child process because the switches are read once per process: both
+    forms against the oracle's FP64 trajectory from the same links and host
+    momenta."""
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
