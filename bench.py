@@ -170,6 +170,54 @@ def delta_dH(S, L, nsteps, beta, seed, ntraj, ntherm=50, nchain=100):
     }
 
 
+def precision_floor(S, Ls, nsteps, beta, seed, ntraj, ntherm=30):
+    """dH of the FP32 form and of the FP64 form with emulated FP32 storage
+    (links, momenta, both rounded to FP32 after every update; FP64
+    arithmetic) against the plain FP64 form, per lattice size: paired OMF2
+    trajectories with the reversibility test from the same equilibrium links
+    and the same Philox momentum stream.  Shows which stored field sets the
+    FP32 form's dH floor and how the floor grows with the volume."""
+    import statistics
+    out = []
+    forms = (("fp32", S.FP32, None), ("fp32_links", S.FP64, (True, False)),
+             ("fp32_momenta", S.FP64, (False, True)), ("fp32_both", S.FP64, (True, True)))
+    for L in Ls:
+        dims = (L, L, L, L, 4)
+        progress(f"precision floor at {L}^4")
+        l64 = S.Lattice(S.SU2, dims, beta, precision=S.FP64)
+        l64.haarstart(seed)
+        for i in range(ntherm):
+            l64.md_update(seed + 100 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE)
+        lats = {}
+        for name, prec, emu in forms:
+            lats[name] = S.Lattice(S.SU2, dims, beta, precision=prec)
+            if emu:
+                lats[name].set_fp32_storage_emulation(*emu)
+        rows = {name: [] for name, _, _ in forms}
+        ref = []
+        for i in range(ntraj):
+            x = l64.get_links()
+            for name, _, _ in forms:
+                lats[name].set_links(x)
+                r = lats[name].md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE,
+                                         revtest=True)
+                rows[name].append(r)
+            ref.append(l64.md_update(seed + 200 + i, S.OMF2, nsteps, momenta=S.MOMENTA_DEVICE,
+                                     revtest=True))
+        for lat in lats.values():
+            lat.close()
+        l64.close()
+        e = {"lattice": [L] * 4, "paired_trajectories": ntraj,
+             "mean_abs_dH64": statistics.fmean(abs(r["deltaH"]) for r in ref),
+             "max_abs_ddH_fp64": max(abs(r["deltadeltaH"]) for r in ref)}
+        for name, _, _ in forms:
+            d = [abs(a["deltaH"] - b["deltaH"]) for a, b in zip(rows[name], ref)]
+            e[name] = {"max_abs_dH_minus_dH64": max(d), "mean_abs_dH_minus_dH64": statistics.fmean(d),
+                       "max_abs_ddH": max(abs(r["deltadeltaH"]) for r in rows[name])}
+        out.append(e)
+    return out
+
+
 def batch_ensemble(S, steps, warmup):
     """The mixed ensemble of 224 lattices (128 U(1) d=2, 64 SU(2) d=3, 32 SU(2)
     d=4), FP32, one launch per MD tick, hipGraph replay, autotuned launch shape."""
@@ -407,6 +455,9 @@ def main():
     ap.add_argument("--no-48", action="store_true", help="skip the 48^4 leg")
     ap.add_argument("--no-metropolis", action="store_true", help="skip the Metropolis-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--precision-scan", type=str, default="",
+                    help="lattice extents of the FP32 precision-floor study, e.g. 8,16,32,48")
+    ap.add_argument("--precision-traj", type=int, default=6)
     ap.add_argument("--scan", type=str, default="",
                     help="comma-separated extents L: also time both forms on L^4 (scan entry)")
     ap.add_argument("--cpu-sample", type=int, default=0, help=argparse.SUPPRESS)
@@ -449,6 +500,10 @@ def main():
     batch = None if args.no_batch else batch_ensemble(S, args.batch_steps, 2)
     flow = None if args.no_flow else flow_adjoint(S)
     metro = None if args.no_metropolis else metropolis(S, args.L)
+    pfloor = None
+    if args.precision_scan:
+        pfloor = precision_floor(S, [int(x) for x in args.precision_scan.split(",")], args.nsteps,
+                                 args.prec_beta, args.seed, args.precision_traj)
     scan = []
     for Ls in filter(None, args.scan.split(",")):
         L = int(Ls)
@@ -505,6 +560,8 @@ def main():
     }
     if scan:
         out["scan"] = scan
+    if pfloor:
+        out["precision_floor"] = pfloor
     print(json.dumps(out), flush=True)
 
 

@@ -104,6 +104,15 @@ int su2gpu_get_momenta(su2gpu_lattice *h, double *dst);
 int su2gpu_flip_momenta(su2gpu_lattice *h);
 /* initnormal(std::mt19937(seed), momenta) (adjointfield.hh:240-258), host stream */
 int su2gpu_initnormal(su2gpu_lattice *h, uint64_t seed);
+/* FP32 storage emulation of an FP64 handle (no counterpart in the reference;
+ * a numerical study in the manner of its rounding integrators lp_leapfrog /
+ * round_and_update_momenta, integrator.hh:53-80, update_momenta.hh:35-59):
+ * links != 0: every updated link, the start links and the restored final links
+ * of md_trajectory / md_update rounded to the nearest FP32 value; momenta != 0:
+ * every updated momentum and the start momenta likewise.  Arithmetic stays
+ * FP64 and the generic MD kernels run (the specialised step does not round).
+ * Both 0 (the default) switch it off.  EINVAL for an FP32 handle. */
+int su2gpu_set_fp32_storage_emulation(su2gpu_lattice *h, int links, int momenta);
 
 /* ---- start configurations (host RNG, the reference's exact draw order) ------ */
 /* hotstart(U, seed, delta) (gaugeconfig.hh:277-302) */

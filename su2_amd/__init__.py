@@ -60,6 +60,7 @@ _SIGS = {
     "su2gpu_set_momenta": (C.c_int, [_h, _vp]),
     "su2gpu_get_momenta": (C.c_int, [_h, _vp]),
     "su2gpu_flip_momenta": (C.c_int, [_h]),
+    "su2gpu_set_fp32_storage_emulation": (C.c_int, [_h, C.c_int, C.c_int]),
     "su2gpu_initnormal": (C.c_int, [_h, C.c_uint64]),
     "su2gpu_hotstart": (C.c_int, [_h, C.c_uint64, C.c_double]),
     "su2gpu_coldstart": (C.c_int, [_h]),
@@ -233,6 +234,11 @@ class Lattice:
 
     def flip_momenta(self):
         _ck(self.lib.su2gpu_flip_momenta(self._h))
+
+    def set_fp32_storage_emulation(self, links, momenta):
+        """FP64 form with the updated links and / or momenta rounded to FP32
+        (a precision study; generic MD kernels)."""
+        _ck(self.lib.su2gpu_set_fp32_storage_emulation(self._h, int(bool(links)), int(bool(momenta))))
 
     def initnormal(self, seed):
         _ck(self.lib.su2gpu_initnormal(self._h, seed))

@@ -84,6 +84,9 @@ struct su2gpu_lattice {
   // split md_update state
   bool pending = false;
   int pend_old = -1, pend_new = -1;
+  // FP32 storage emulation of the FP64 form (su2gpu_set_fp32_storage_emulation):
+  // bit 0 links, bit 1 momenta
+  int r32 = 0;
   // profiling
   bool prof = false;
   struct Rec {
@@ -456,6 +459,8 @@ int run_integrator(L *h, int kind, size_t nsteps, double tau, size_t exponent,
     c0.round_n = (double)(size_t)std::pow(10, (double)exponent);
   }
   if (h->prec == 1 && h->group == 1) c0.flags |= su2k::F_RESTORE;  // FP32: reunitarise every step
+  if (h->r32 & 1) c0.flags |= su2k::F_R32L;
+  if (h->r32 & 2) c0.flags |= su2k::F_R32P;
   int cur = h->cur;
   auto is_prot = [&](int b) {
     for (int p : protect)
@@ -832,6 +837,13 @@ int su2gpu_get_momenta(su2gpu_lattice *h, double *dst) {
   if (!dst) return fail(SU2GPU_EINVAL, "null destination");
   return get_mom_host(h, h->P, dst);
 }
+int su2gpu_set_fp32_storage_emulation(su2gpu_lattice *h, int links, int momenta) {
+  CK(check(h));
+  if (h->prec != SU2GPU_FP64) return fail(SU2GPU_EINVAL, "FP32 storage emulation needs the FP64 form");
+  if (h->pending) return fail(SU2GPU_ESTATE, "md_trajectory pending: call su2gpu_md_finish first");
+  h->r32 = (links ? 1 : 0) | (momenta ? 2 : 0);
+  return SU2GPU_OK;
+}
 int su2gpu_flip_momenta(su2gpu_lattice *h) {
   CK(check(h));
   {
@@ -1057,6 +1069,10 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
   const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
   const int pflags = h->aniso ? su2k::F_ANISO : 0;
   h->cur = u_start;
+  // FP32 storage emulation: the fields an FP32 store would hold at the start
+  if (h->r32 & 1) su2k::launch_round_f32((double *)h->U[u_start], (size_t)h->nlinks * h->ncomp, h->stream);
+  if (h->r32 & 2) su2k::launch_round_f32((double *)h->P, (size_t)h->nlinks * h->nm, h->stream);
+  if (h->r32) CK(last_launch("round_f32"));
   su2k::RowList rl;  // every sum of the trajectory is reduced in one launch at the end
   // the FP32 form takes S_old as the FP64 plaquette sum of its FP32 links
   if (h->prec == 1) CK(plaq_launch(h, u_start, -1, w_t, w_s, 0, pflags, rbase + 6, &rl));
@@ -1074,6 +1090,12 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
                 &out));
     CK(plaq_launch(h, cur, out, w_t, w_s, 0, pflags | su2k::F_RESTORE, rbase + 7, &rl));
     cur = out;
+    if (h->r32 & 1) {
+      // the restored links as FP32 would store them, and their action
+      su2k::launch_round_f32((double *)h->U[cur], (size_t)h->nlinks * h->ncomp, h->stream);
+      CK(last_launch("round_f32"));
+      CK(plaq_launch(h, cur, -1, w_t, w_s, 0, pflags, rbase + 7, &rl));
+    }
   } else {
     CK(plaq_launch(h, cur, -1, w_t, w_s, 0, pflags, rbase + 7, &rl));
   }

@@ -524,7 +524,7 @@ static bool fast_geo_for(const Geo &g, FastGeo &f, bool wmu = false) {
 bool fused_half_ok(int group, int prec, const Geo &g, int flags) {
   if (group != 1 || prec != 0 || g.nd != 4) return false;
   if (fast_level() < 2 || fast_shape() == 1 || half_pref() == 0) return false;
-  if (flags & (F_PZERO | F_ROUND | F_ANISO)) return false;
+  if (flags & (F_PZERO | F_ROUND | F_ANISO | F_R32L | F_R32P)) return false;
   if (fused_mode(g) == 0) return false;
   FastGeo f;
   return fast_geo_for(g, f);
@@ -538,7 +538,7 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
   } else {
     const int lvl = fast_level();
     if (lvl == 0 || (sizeof(T) == 8 && lvl < 2)) return false;
-    if (L.c.flags & (F_PZERO | F_ROUND)) return false;
+    if (L.c.flags & (F_PZERO | F_ROUND | F_R32L | F_R32P)) return false;
     // the thread-per-site shape keeps its own partial-sum count (fused_blocks)
     if (fused_mode(L.geo) == 0) return false;
     FastGeo f;
@@ -685,7 +685,7 @@ static bool launch_fast_pout(const FusedLaunch &L, hipStream_t st) {
   } else {
     const int lvl = fast_level();
     if (lvl == 0 || (sizeof(T) == 8 && lvl < 2)) return false;
-    if (L.c.flags & (F_PZERO | F_ROUND | F_ACCUM | F_ANISO)) return false;
+    if (L.c.flags & (F_PZERO | F_ROUND | F_ACCUM | F_ANISO | F_R32L | F_R32P)) return false;
     if (fused_mode(L.geo) == 0 || fast_shape() == 1) return false;
     FastGeo f;
     const int xw = xwalk_pref();
@@ -795,6 +795,15 @@ void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int
     if (prec == 0) launch_update_gauge_g<GU1<double>>(U, P, V, nd, eps, round_n, flags, st);
     else launch_update_gauge_g<GU1<float>>(U, P, V, nd, eps, round_n, flags, st);
   }
+}
+
+__global__ __launch_bounds__(NT) void k_round_f32(double *__restrict__ x, size_t n) {
+  const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (i < n) x[i] = (double)(float)x[i];
+}
+void launch_round_f32(double *x, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_round_f32, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, x, n);
 }
 
 void launch_restore(int group, int prec, void *U, int n, hipStream_t st) {

@@ -22,6 +22,8 @@ enum : int {
   F_COMPENSATED = 8,   // TwoSum-compensated staple accumulation
   F_ROUND = 16,        // lp_* integrators: round force and link (update_*.hh round variants)
   F_ANISO = 32,        // anisotropic staple / plaquette weights
+  F_R32L = 64,         // FP32 storage emulation: updated links rounded to FP32
+  F_R32P = 128,        // FP32 storage emulation: updated momenta rounded to FP32
 };
 
 struct FusedCoef {
@@ -62,6 +64,8 @@ int fused_mode(const Geo &g);    // 0 thread per site, 1 thread per link, 2 link
 void launch_update_gauge(int group, int prec, void *U, const void *P, int V, int nd, double eps,
                          double round_n, int flags, hipStream_t st);
 void launch_restore(int group, int prec, void *U, int n, hipStream_t st);
+// every element of an FP64 array rounded to the nearest FP32 value, in place
+void launch_round_f32(double *x, size_t n, hipStream_t st);
 // returns the number of block partials written (one row, at most fused_blocks(g));
 // in_half: Uin in the half-field layout (where fused_half_ok holds), Uout always
 // in the record layout

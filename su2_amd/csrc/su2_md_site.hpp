@@ -13,6 +13,12 @@ namespace su2k {
 
 using namespace su2dev;
 
+// FP32 storage emulation (F_R32L / F_R32P): the value an FP32 field would hold
+template <typename T> __device__ __forceinline__ T r32(T x) { return T(float(x)); }
+template <typename T> __device__ __forceinline__ quat<T> r32(const quat<T> &q) {
+  return quat<T>{T(float(q.ar)), T(float(q.ai)), T(float(q.br)), T(float(q.bi))};
+}
+
 template <class G, int ND, bool WRITE_U>
 __device__ __forceinline__ void md_site(const typename G::link *__restrict__ Uin,
                                         typename G::link *__restrict__ Uout,
@@ -65,6 +71,7 @@ __device__ __forceinline__ void md_site(const typename G::link *__restrict__ Uin
       T f = coef * d[k];
       if (c.flags & F_ROUND) f = ::round(f * rn) / rn;
       p[k] -= ep * f;
+      if (c.flags & F_R32P) p[k] = r32(p[k]);
       kin_new += double(p[k]) * double(p[k]);
       P[(mu * NM + k) * V + s] = p[k];
     }
@@ -73,6 +80,7 @@ __device__ __forceinline__ void md_site(const typename G::link *__restrict__ Uin
       if (c.flags & F_ROUND) u = G::round(u, rn);
       link un = G::expmul(p, eu, u);
       if (c.flags & F_RESTORE) un = G::restore(un);
+      if (c.flags & F_R32L) un = r32(un);
       Uout[mu * V + s] = un;
     }
   }
@@ -169,6 +177,7 @@ __device__ __forceinline__ void md_link(const typename G::link *__restrict__ Uin
     T f = coef * d[k];
     if (c.flags & F_ROUND) f = ::round(f * rn) / rn;
     p[k] -= ep * f;
+    if (c.flags & F_R32P) p[k] = r32(p[k]);
     kin_new += double(p[k]) * double(p[k]);
     P[(mu * NM + k) * V + s] = p[k];
   }
@@ -177,6 +186,7 @@ __device__ __forceinline__ void md_link(const typename G::link *__restrict__ Uin
     if (c.flags & F_ROUND) uu = G::round(uu, rn);
     link un = G::expmul(p, eu, uu);
     if (c.flags & F_RESTORE) un = G::restore(un);
+    if (c.flags & F_R32L) un = r32(un);
     Uout[mu * V + s] = un;
   }
 }
