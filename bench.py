@@ -220,7 +220,17 @@ def precision_floor(S, Ls, nsteps, beta, seed, ntraj, ntherm=30):
 
 def batch_ensemble(S, steps, warmup):
     """The mixed ensemble of 224 lattices (128 U(1) d=2, 64 SU(2) d=3, 32 SU(2)
-    d=4), FP32, one launch per MD tick, hipGraph replay, autotuned launch shape."""
+    d=4), FP32, one launch per part and MD tick, hipGraph replay, autotuned
+    launch shape.  gpu_busy_fraction: kernel time (HIP events around each
+    launch group of an eager trajectory, so each includes a little launch
+    latency) over the graph replay's wall time per trajectory, which also holds
+    the launch gaps, the host's descriptor upload, sums copy and Metropolis;
+    launch_gap_ms_per_traj: what the wall time exceeds the kernel time by
+    (0 when the event overhead covers it); per part the ticks' link
+    updates/s, algorithmic bandwidth (84 B per SU(2), 24 B per U(1) link
+    update, FP32) and the active tile fraction (lattices with fewer steps idle
+    in the later ticks)."""
+    import statistics
     members = S.ensemble_members()
     b = S.Batch(members, precision=S.FP32)
     import tempfile
@@ -239,11 +249,31 @@ def batch_ensemble(S, steps, warmup):
         acc += sum(r["accept"] for r in res)
     dt = time.perf_counter() - t0
     lu = b.link_updates()
+    # launch gaps and per-part efficiency: eager trajectories with HIP events
+    # around every launch group (after the timed graph replays)
+    prof = [b.profile(20_000 + i) for i in range(3)]
     b.close()
+    ms_graph = 1000 * dt / steps
+    kern = statistics.fmean(p["ms_momenta"] + p["ms_plaq"] + sum(p["ms_tick"].values()) +
+                            p["ms_restore"] + p["ms_reduce"] for p in prof)
+    parts = {}
+    for part, nbytes in (("su2_d4", 84), ("su2_d3", 84), ("generic", 24)):
+        ms = statistics.fmean(p["ms_tick"][part] for p in prof)
+        n = prof[0]["link_updates"][part]
+        parts[part] = {"tick_ms_per_traj": ms, "link_updates_per_traj": n,
+                       "link_updates_per_s": n / (ms * 1e-3) if ms > 0 else None,
+                       "algorithmic_GBs": n * nbytes / (ms * 1e-3) / 1e9 if ms > 0 else None,
+                       "roofline_frac": n * nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS if ms > 0 else None,
+                       "bytes_per_link_update": nbytes,
+                       "active_tile_fraction": prof[0]["active_tile_fraction"][part]}
     return {"lattices": len(members), "link_updates_per_traj": lu,
-            "link_updates_per_s": lu * steps / dt, "ms_per_traj": 1000 * dt / steps,
+            "link_updates_per_s": lu * steps / dt, "ms_per_traj": ms_graph,
             "acceptance": acc / (steps * len(members)), "block": blk, "grid": grd,
-            "autotune_s": round(t_tune, 3), "graph": True, "precision": "fp32"}
+            "autotune_s": round(t_tune, 3), "graph": True, "precision": "fp32",
+            "kernel_ms_per_traj": kern, "gpu_busy_fraction": kern / ms_graph,
+            "launch_gap_ms_per_traj": max(0.0, ms_graph - kern),
+            "eager_ms_per_traj": statistics.fmean(p["ms_total"] for p in prof),
+            "parts": parts}
 
 
 def flow_adjoint(S, L=24, beta=2.4, eps=0.01, tmax=2.0, ckpt=20, ntherm=50, seed=4242):

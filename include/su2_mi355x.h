@@ -273,6 +273,14 @@ int su2gpu_batch_autotune(su2gpu_batch *b, const char *cache_path, int *block, i
 int su2gpu_batch_use_graph(su2gpu_batch *b, int on);
 /* sum over members of nd * V * nsteps: the link-updates of one batch trajectory */
 int su2gpu_batch_link_updates(const su2gpu_batch *b, double *per_trajectory);
+/* launch-gap and per-part efficiency (no counterpart in the reference, which
+ * has no batch): one eager trajectory of device momenta (with Metropolis, as
+ * su2gpu_batch_trajectory) timed with HIP events around every launch group.
+ * out[0..7] ms: whole trajectory, momenta, plaquette sums, MD ticks of the
+ * SU(2) d = 4 part, of the d = 3 part, of the generic rest, restore, reduction;
+ * out[8..10] link updates of the three parts; out[11..13] their active tile
+ * fraction.  nout >= 14. */
+int su2gpu_batch_profile(su2gpu_batch *b, uint64_t traj, double *out, size_t nout);
 
 /* ---- configuration files (gaugeconfig.hh:225-252) ------------------------------ */
 int su2gpu_save(su2gpu_lattice *h, const char *path);

@@ -121,6 +121,7 @@ _SIGS.update({
     "su2gpu_batch_autotune": (C.c_int, [_h, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "su2gpu_batch_use_graph": (C.c_int, [_h, C.c_int]),
     "su2gpu_batch_link_updates": (C.c_int, [_h, _dp]),
+    "su2gpu_batch_profile": (C.c_int, [_h, C.c_uint64, _dp, C.c_size_t]),
 })
 
 
@@ -470,6 +471,17 @@ class Batch:
         v = C.c_double()
         _ck(self.lib.su2gpu_batch_link_updates(self._h, C.byref(v)))
         return v.value
+
+    def profile(self, traj):
+        """One eager trajectory timed per launch group (su2gpu_batch_profile)."""
+        out = (C.c_double * 14)()
+        _ck(self.lib.su2gpu_batch_profile(self._h, traj, out, 14))
+        v = list(out)
+        parts = ("su2_d4", "su2_d3", "generic")
+        return {"ms_total": v[0], "ms_momenta": v[1], "ms_plaq": v[2],
+                "ms_tick": dict(zip(parts, v[3:6])), "ms_restore": v[6], "ms_reduce": v[7],
+                "link_updates": dict(zip(parts, v[8:11])),
+                "active_tile_fraction": dict(zip(parts, v[11:14]))}
 
 
 def ensemble_members(seed=13526463):

@@ -157,24 +157,76 @@ int fin_arena(const su2k::BLat &L) {
   return ((L.npairs - 1) & 1) ? a2 : a1;
 }
 
-// the launches of one trajectory (captured into a graph when use_graph)
-int enqueue(B *b, bool devmom) {
+// HIP events around the launch groups of one eager trajectory
+// (su2gpu_batch_profile): kinds of su2gpu_batch_profile's output
+struct Rec {
+  std::vector<int> kind;
+  std::vector<hipEvent_t> ev;  // two per entry
+  hipStream_t st = nullptr;
+  void begin(int k) {
+    hipEvent_t a, e;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(a, st);
+    kind.push_back(k);
+    ev.push_back(a);
+    ev.push_back(e);
+  }
+  void end() { (void)hipEventRecord(ev.back(), st); }
+  ~Rec() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
+enum { RK_MOMENTA = 0, RK_PLAQ = 1, RK_TICK4 = 2, RK_TICK3 = 3, RK_TICK0 = 4, RK_RESTORE = 5,
+       RK_REDUCE = 6, RK_N = 7 };
+
+// the launches of one trajectory (captured into a graph when use_graph;
+// rec: eager, each launch group between two events)
+int enqueue(B *b, bool devmom, Rec *rec = nullptr) {
   const int n = (int)b->lat.size();
-  if (devmom)
+  auto beg = [&](int k) {
+    if (rec) rec->begin(k);
+  };
+  auto fin = [&]() {
+    if (rec) rec->end();
+  };
+  if (devmom) {
+    beg(RK_MOMENTA);
     su2k::launch_batch_momenta(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->dtraj,
                                b->grid_site, b->block, b->stream);
+    fin();
+  }
+  beg(RK_PLAQ);
   su2k::launch_batch_plaq(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, 0, 4, b->nltiles,
                           b->partials, b->grid_site, b->block, b->stream);
+  fin();
   const int flags = (b->prec == 1) ? (su2k::F_COMPENSATED | su2k::F_RESTORE) : 0;
-  for (int t = 0; t <= b->maxpairs; ++t)
-    su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3, b->ar,
-                            t, flags, b->partials, b->grid, b->block, b->stream);
-  if (b->prec == 0)
+  for (int t = 0; t <= b->maxpairs; ++t) {
+    if (!rec) {
+      su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
+                              b->ar, t, flags, b->partials, b->grid, b->block, b->stream);
+      continue;
+    }
+    for (int part = 0; part < 3; ++part) {
+      beg(RK_TICK4 + part);
+      su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
+                              b->ar, t, flags, b->partials, b->grid, b->block, b->stream, part);
+      fin();
+    }
+  }
+  if (b->prec == 0) {
+    beg(RK_RESTORE);
     su2k::launch_batch_restore(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, b->grid_site,
                                b->block, b->stream);
+    fin();
+  }
+  beg(RK_PLAQ);
   su2k::launch_batch_plaq(b->prec, b->dlat, b->dtiles, b->ntiles, b->ar, 1, 3, b->nltiles,
                           b->partials, b->grid_site, b->block, b->stream);
+  fin();
+  beg(RK_REDUCE);
   su2k::launch_batch_reduce(b->dlat, n, b->partials, b->nltiles, b->dsums, b->stream);
+  fin();
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return bfail(SU2GPU_EHIP, "batch launch: %s", hipGetErrorString(e));
   BH(hipMemcpyAsync(b->hsums, b->dsums, (size_t)n * 2 * NSUM * sizeof(double),
@@ -186,12 +238,15 @@ long double bsum(const B *b, int l, int r) {
   return (long double)b->hsums[l * 2 * NSUM + 2 * r] + (long double)b->hsums[l * 2 * NSUM + 2 * r + 1];
 }
 
-int run_trajectory(B *b, bool devmom, uint64_t traj, int metropolis, su2gpu_md_result *res) {
+int run_trajectory(B *b, bool devmom, uint64_t traj, int metropolis, su2gpu_md_result *res,
+                   Rec *rec = nullptr) {
   *b->htraj = traj;
   BH(hipMemcpyAsync(b->dtraj, b->htraj, sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
   for (auto &L : b->lat) L.fin = fin_arena(L);
   BK(upload_desc(b));
-  if (b->use_graph) {
+  if (rec) {
+    BK(enqueue(b, devmom, rec));
+  } else if (b->use_graph) {
     if (!b->gexec || b->graph_devmom != (int)devmom) {
       if (b->gexec) BH(hipGraphExecDestroy(b->gexec));
       if (b->graph) BH(hipGraphDestroy(b->graph));
@@ -442,6 +497,44 @@ int su2gpu_batch_get_launch(const su2gpu_batch *b, int *block, int *grid) {
 int su2gpu_batch_use_graph(su2gpu_batch *b, int on) {
   if (!b) return bfail(SU2GPU_EINVAL, "null batch");
   b->use_graph = on != 0;
+  return SU2GPU_OK;
+}
+
+// One trajectory of device momenta (Metropolis as su2gpu_batch_trajectory),
+// eager, with HIP events around every launch group.  out[0 .. 7]: ms of the
+// whole trajectory (first to last event), of the momentum draw, the two
+// plaquette sums, the MD ticks of the SU(2) d = 4 part, of the d = 3 part, of
+// the remaining (generic) part, the final restore and the reduction;
+// out[8 .. 10]: link updates of the three parts; out[11 .. 13]: their active
+// tile fraction (tile-ticks of lattices not yet finished over all tile-ticks
+// launched: lattices with fewer steps idle in the later ticks).
+int su2gpu_batch_profile(su2gpu_batch *b, uint64_t traj, double *out, size_t nout) {
+  if (!b || !out) return bfail(SU2GPU_EINVAL, "null pointer");
+  if (nout < 14) return bfail(SU2GPU_EINVAL, "batch profile: out needs 14 entries");
+  Rec rec;
+  rec.st = b->stream;
+  std::vector<su2gpu_md_result> res(b->lat.size());
+  BK(run_trajectory(b, true, traj, 1, res.data(), &rec));
+  for (size_t i = 0; i < 14; ++i) out[i] = 0.0;
+  for (size_t i = 0; i < rec.kind.size(); ++i) {
+    float ms = 0.f;
+    BH(hipEventElapsedTime(&ms, rec.ev[2 * i], rec.ev[2 * i + 1]));
+    out[1 + rec.kind[i]] += ms;
+  }
+  if (!rec.ev.empty()) {
+    float ms = 0.f;
+    BH(hipEventElapsedTime(&ms, rec.ev.front(), rec.ev.back()));
+    out[0] = ms;
+  }
+  double act[3] = {0, 0, 0}, all[3] = {0, 0, 0};
+  for (size_t i = 0; i < b->lat.size(); ++i) {
+    const su2k::BLat &L = b->lat[i];
+    const int part = L.spec == 4 ? 0 : L.spec == 3 ? 1 : 2;
+    out[8 + part] += (double)L.geo.V * L.geo.nd * (double)b->mem[i].nsteps;
+    act[part] += (double)L.nltiles * (L.npairs + 1);
+    all[part] += (double)L.nltiles * (b->maxpairs + 1);
+  }
+  for (int k = 0; k < 3; ++k) out[11 + k] = all[k] > 0 ? act[k] / all[k] : 0.0;
   return SU2GPU_OK;
 }
 

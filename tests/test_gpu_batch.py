@@ -106,6 +106,39 @@ def test_graph_replay_matches_eager(S, oracle):
             np.testing.assert_array_equal(a, c)
 
 
+def test_profile_trajectory_matches_trajectory(S, oracle):
+    """The profiled trajectory (eager, per-part launches between HIP events)
+    leaves every member's links exactly as the ordinary trajectory of the same
+    number does, and reports consistent part sizes and times."""
+    starts = _starts(oracle)
+    links = []
+    prof = None
+    for profiled in (False, True):
+        with S.Batch(MIXED, precision=1) as b:
+            for i, (U, _) in enumerate(starts):
+                b.set_links(i, U)
+            if profiled:
+                prof = b.profile(77)
+                total = b.link_updates()
+            else:
+                b.trajectory(77, momenta=S.MOMENTA_DEVICE)
+            links.append([b.get_links(i).copy() for i in range(len(MIXED))])
+    for a, c in zip(*links):
+        np.testing.assert_array_equal(a, c)
+    print(prof)
+    assert sum(prof["link_updates"].values()) == total
+    # MIXED: SU(2) d = 4 members 2, 5, 7; d = 3 members 1, 6; U(1) and the
+    # SU(2) d = 2 member the generic part
+    n4 = sum(np.prod(m["dims"][:4]) * 4 * m["nsteps"] for m in MIXED if m["group"] == 1 and m["dims"][4] == 4)
+    assert prof["link_updates"]["su2_d4"] == n4
+    for part in ("su2_d4", "su2_d3", "generic"):
+        assert prof["ms_tick"][part] > 0
+        assert 0 < prof["active_tile_fraction"][part] <= 1
+    parts_ms = (prof["ms_momenta"] + prof["ms_plaq"] + sum(prof["ms_tick"].values()) +
+                prof["ms_restore"] + prof["ms_reduce"])
+    assert 0 < parts_ms <= prof["ms_total"] * 1.001
+
+
 def test_autotune_cache_and_no_side_effects(S, oracle, tmp_path):
     starts = _starts(oracle)
     cache = str(tmp_path / "tune.json")
