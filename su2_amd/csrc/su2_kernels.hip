@@ -467,7 +467,8 @@ static int xwalk_pref() {
 // SU2GPU_NT = 1 / 0: non-temporal stores of the new links and momenta in the
 // specialised step; default: where the two link fields and the momenta exceed
 // the 256 MB Infinity Cache (48^4 FP32 466 against 496 us, 64^4 1331 against
-// 1446 us; at 32^4, which fits, 79 against 67 us)
+// 1446 us; at 32^4, which fits, 79 against 67 us).  2: also non-temporal
+// momentum loads in the FP64 half-field step (default beyond 1 GiB)
 static int nt_pref() {
   static int v = -2;
   if (v == -2) v = env_int("SU2GPU_NT", -1);
@@ -499,17 +500,24 @@ static int zs_pref() {
 // The specialised step on FP64 links in the half-field layout: input UI
 // (QHalf or a record-layout pointer), output QHalfW (a P-only step passes a
 // null QHalfW it never writes)
+// nt: 0 ordinary stores, 1 non-temporal stores, 2 also non-temporal
+// momentum loads
 template <bool W, bool A, class UI>
-static void launch_half(bool zs, bool nt, UI uin, QHalfW uout, const FusedLaunch &L,
+static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch &L,
                         const FastGeo &f, hipStream_t st) {
   const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
 #define SU2_HALF(Z, N)                                                                          \
   hipLaunchKernelGGL((k_md_su2_4d<double, W, A, 1, Z, false, N, UI, QHalfW>), grid, block, 0, st, \
                      uin, uout, (double *)L.P, f, L.c, L.partials, L.pstride)
-  if (zs && nt) SU2_HALF(true, true);
-  else if (zs) SU2_HALF(true, false);
-  else if (nt) SU2_HALF(false, true);
-  else SU2_HALF(false, false);
+  if (zs) {
+    if (nt == 2) SU2_HALF(1, 2);
+    else if (nt == 1) SU2_HALF(1, 1);
+    else SU2_HALF(1, 0);
+  } else {
+    if (nt == 2) SU2_HALF(0, 2);
+    else if (nt == 1) SU2_HALF(0, 1);
+    else SU2_HALF(0, 0);
+  }
 #undef SU2_HALF
 }
 
@@ -614,6 +622,9 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     if (L.in_half || L.out_half) {
       // the host asked for it only where fused_half_ok holds
       if constexpr (sizeof(T) == 8) {
+        // momenta loaded non-temporally too where the fields exceed 1 GiB
+        // (48^4: 832 against 850 us; 32^4, 368 MB: 161 against 148 us)
+        const int ntl = nt ? ((ntp < 0 && fields > 1024.0 * 1024 * 1024) || ntp == 2 ? 2 : 1) : 0;
         const size_t nl = 4 * (size_t)L.geo.V;
         const su2_d2 *ia = (const su2_d2 *)L.Uin;
         su2_d2 *oa = (su2_d2 *)L.Uout;
@@ -621,14 +632,14 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
         if (L.Uout && !L.out_half) return false;  // half -> record steps are not built
         if (L.in_half) {
           const QHalf hin{ia, ia + nl};
-          if (L.Uout) acc ? launch_half<true, true>(zs, nt, hin, hout, L, f, st)
-                          : launch_half<true, false>(zs, nt, hin, hout, L, f, st);
-          else acc ? launch_half<false, true>(zs, nt, hin, hout, L, f, st)
-                   : launch_half<false, false>(zs, nt, hin, hout, L, f, st);
+          if (L.Uout) acc ? launch_half<true, true>(zs, ntl, hin, hout, L, f, st)
+                          : launch_half<true, false>(zs, ntl, hin, hout, L, f, st);
+          else acc ? launch_half<false, true>(zs, ntl, hin, hout, L, f, st)
+                   : launch_half<false, false>(zs, ntl, hin, hout, L, f, st);
         } else {
           const quat<double> *rin = (const quat<double> *)L.Uin;
-          acc ? launch_half<true, true>(zs, nt, rin, hout, L, f, st)
-              : launch_half<true, false>(zs, nt, rin, hout, L, f, st);
+          acc ? launch_half<true, true>(zs, ntl, rin, hout, L, f, st)
+              : launch_half<true, false>(zs, ntl, rin, hout, L, f, st);
         }
         return true;
       } else {

@@ -698,8 +698,9 @@ print("NT_CHILD_OK", flush=True)
 
 @pytest.mark.parametrize("env", [{"SU2GPU_NT": "1", "SU2GPU_XWALK": "1"},
                                  {"SU2GPU_NT": "1", "SU2GPU_XWALK": "0"},
+                                 {"SU2GPU_NT": "2"},
                                  {"SU2GPU_NT": "0", "SU2GPU_ZG": "1"}],
-                         ids=["nt_xwalk", "nt", "zg"])
+                         ids=["nt_xwalk", "nt", "nt2", "zg"])
 def test_nontemporal_store_step_parity(env):
     """The specialised step with its non-temporal stores (quat<float> and
     quat<double> sto_nt, with the FP32 general z exchange where Lz does not
