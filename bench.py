@@ -443,8 +443,8 @@ def kernel_name(prec, L):
     +-z lane exchange within a wave (64 % L == 0), 2 the general exchange
     (FP32 beyond 256 MiB otherwise), 0 none.  NT: non-temporal stores (1)
     and momentum loads (2: the FP32 general-exchange step, the FP64 step
-    beyond 1 GiB) where the two link fields and the momenta exceed 256 MiB.  The FP64 form steps on the
-    half-field layout (QHalf in, QHalfW out)."""
+    beyond 1 GiB) where the two link fields and the momenta exceed 256 MiB.
+    The FP64 form steps on the half-field layout (QHalf in, QHalfW out)."""
     r = 8 if prec == 0 else 4
     big = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
     if prec == 0:
