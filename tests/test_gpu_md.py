@@ -635,9 +635,9 @@ def test_fp64_and_fp32_trajectory_parity_48(S, oracle):
     both forms against the oracle's FP64 trajectory (md_update.hh:18-81,
     integrator.hh:83-115).  The FP64 form: <= 1e-10 per link, dH and the
     plaquette to 1e-9 relative.  The FP32 form (at 48^4 the step runs with
-    non-temporal stores and without the z lane exchange, 64 % 48 != 0): <= 1e-4
-    per link, dH within the sqrt(N) rounding bound of the FP32 fields, the
-    plaquette sum within 1e-8 relative."""
+    non-temporal stores and momentum loads and the general z lane exchange,
+    64 % 48 != 0): <= 1e-4 per link, dH within the sqrt(N) rounding bound of
+    the FP32 fields, the plaquette sum within 1e-8 relative."""
     dims = (48, 48, 48, 48, 4)
     beta = 2.5
     U = oracle.haarstart(1, dims, SEED)
