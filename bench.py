@@ -220,13 +220,16 @@ def precision_floor(S, Ls, nsteps, beta, seed, ntraj, ntherm=30):
 
 def batch_ensemble(S, steps, warmup):
     """The mixed ensemble of 224 lattices (128 U(1) d=2, 64 SU(2) d=3, 32 SU(2)
-    d=4), FP32, one launch per part and MD tick, hipGraph replay, autotuned
-    launch shape.  gpu_busy_fraction: kernel time (HIP events around each
-    launch group of an eager trajectory, so each includes a little launch
-    latency) over the graph replay's wall time per trajectory, which also holds
-    the launch gaps, the host's descriptor upload, sums copy and Metropolis;
-    launch_gap_ms_per_traj: what the wall time exceeds the kernel time by
-    (0 when the event overhead covers it); per part the ticks' link
+    d=4), FP32, one launch per part and MD tick (the three parts' tick chains
+    on three streams, parallel branches of the replayed hipGraph), autotuned
+    launch shape.  kernel_ms_per_traj: the kernel time of an eager trajectory
+    with the parts one after another (HIP events around each launch group, so
+    each includes a little launch latency); gpu_busy_fraction: that over the
+    graph replay's wall time per trajectory, above 1 by the overlap of the
+    parts (the wall time also holds the launch gaps, the host's descriptor
+    upload, sums copy and Metropolis); launch_gap_ms_per_traj: what the wall
+    time exceeds the kernel time by (0 when the overlap and the event
+    overhead cover it); per part the ticks' link
     updates/s, algorithmic bandwidth (84 B per SU(2), 24 B per U(1) link
     update, FP32) and the active tile fraction (lattices with fewer steps idle
     in the later ticks)."""

@@ -72,6 +72,13 @@ struct su2gpu_batch {
   double *staging = nullptr, *hstage = nullptr;
   size_t stage_cap = 0;
   hipStream_t stream = nullptr;
+  // the d = 3 and generic parts' tick chains run on streams of their own,
+  // forked from and joined back into `stream` (captured as parallel branches
+  // of the trajectory's graph); each part reads and writes only its own
+  // lattices, so the chains are independent
+  hipStream_t pstream[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  bool concurrent = true;
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -201,7 +208,22 @@ int enqueue(B *b, bool devmom, Rec *rec = nullptr) {
                           b->partials, b->grid_site, b->block, b->stream);
   fin();
   const int flags = (b->prec == 1) ? (su2k::F_COMPENSATED | su2k::F_RESTORE) : 0;
-  for (int t = 0; t <= b->maxpairs; ++t) {
+  if (!rec && b->concurrent) {
+    // three independent tick chains: d = 4 on `stream`, d = 3 and the rest on
+    // the part streams, joined before the final sums
+    BH(hipEventRecord(b->ev_fork, b->stream));
+    for (int k = 0; k < 2; ++k) BH(hipStreamWaitEvent(b->pstream[k], b->ev_fork, 0));
+    for (int t = 0; t <= b->maxpairs; ++t)
+      for (int part = 0; part < 3; ++part)
+        su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
+                                b->ar, t, flags, b->partials, b->grid, b->block,
+                                part == 0 ? b->stream : b->pstream[part - 1], part);
+    for (int k = 0; k < 2; ++k) {
+      BH(hipEventRecord(b->ev_join[k], b->pstream[k]));
+      BH(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
+    }
+  }
+  for (int t = 0; t <= b->maxpairs && (rec || !b->concurrent); ++t) {
     if (!rec) {
       su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
                               b->ar, t, flags, b->partials, b->grid, b->block, b->stream);
@@ -367,6 +389,16 @@ int su2gpu_batch_create(su2gpu_batch **out, const su2gpu_batch_member *members, 
     b->lat.push_back(L);
   }
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k)
+    e = hipStreamCreateWithFlags(&b->pstream[k], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+  for (int k = 0; k < 2 && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&b->ev_join[k], hipEventDisableTiming);
+  {
+    // SU2GPU_BATCH_CONCURRENT = 0: the three parts one after another on one stream
+    const char *ev = std::getenv("SU2GPU_BATCH_CONCURRENT");
+    b->concurrent = !(ev && std::atoi(ev) == 0);
+  }
   for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc(&b->ar.U[k], b->ubytes);
   if (e == hipSuccess) e = hipMalloc(&b->ar.P, b->pelems * rb);
   if (e == hipSuccess) e = hipMalloc(&b->dlat, n * sizeof(su2k::BLat));
@@ -406,6 +438,8 @@ int su2gpu_batch_create(su2gpu_batch **out, const su2gpu_batch_member *members, 
 int su2gpu_batch_destroy(su2gpu_batch *b) {
   if (!b) return SU2GPU_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
+  for (hipStream_t ps : b->pstream)
+    if (ps) hipStreamSynchronize(ps);
   if (b->gexec) hipGraphExecDestroy(b->gexec);
   if (b->graph) hipGraphDestroy(b->graph);
   for (int k = 0; k < 3; ++k)
@@ -415,6 +449,10 @@ int su2gpu_batch_destroy(su2gpu_batch *b) {
     if (p) hipFree(p);
   for (void *p : {(void *)b->hsums, (void *)b->htraj, (void *)b->hstage})
     if (p) hipHostFree(p);
+  for (hipStream_t ps : b->pstream)
+    if (ps) hipStreamDestroy(ps);
+  for (hipEvent_t ev : {b->ev_fork, b->ev_join[0], b->ev_join[1]})
+    if (ev) hipEventDestroy(ev);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return SU2GPU_OK;
