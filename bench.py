@@ -231,8 +231,8 @@ def batch_ensemble(S, steps, warmup):
     time exceeds the kernel time by (0 when the overlap and the event
     overhead cover it); per part the ticks' link
     updates/s, algorithmic bandwidth (84 B per SU(2), 24 B per U(1) link
-    update, FP32) and the active tile fraction (lattices with fewer steps idle
-    in the later ticks)."""
+    update, FP32) and the active tile fraction (the share of a full launch per
+    tick that lattices still running need; the launches cover only that)."""
     import statistics
     members = S.ensemble_members()
     b = S.Batch(members, precision=S.FP32)

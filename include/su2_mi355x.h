@@ -279,7 +279,8 @@ int su2gpu_batch_link_updates(const su2gpu_batch *b, double *per_trajectory);
  * out[0..7] ms: whole trajectory, momenta, plaquette sums, MD ticks of the
  * SU(2) d = 4 part, of the d = 3 part, of the generic rest, restore, reduction;
  * out[8..10] link updates of the three parts; out[11..13] their active tile
- * fraction.  nout >= 14. */
+ * fraction (tile-ticks of running lattices over tiles x ticks; the tick
+ * launches cover only the running lattices' tiles).  nout >= 14. */
 int su2gpu_batch_profile(su2gpu_batch *b, uint64_t traj, double *out, size_t nout);
 
 /* ---- configuration files (gaugeconfig.hh:225-252) ------------------------------ */

@@ -67,6 +67,8 @@ struct su2gpu_batch {
   int2 *dtiles = nullptr, *dltiles = nullptr;
   int ntiles = 0, nltiles = 0, nspec4 = 0, nspec3 = 0, block = 256, grid = 0, grid_site = 0,
       maxpairs = 0;
+  // active[3 t + k]: leading link tiles of part k whose lattices still run at tick t
+  std::vector<int> active;
   double *partials = nullptr, *dsums = nullptr, *hsums = nullptr;
   uint64_t *dtraj = nullptr, *htraj = nullptr;
   double *staging = nullptr, *hstage = nullptr;
@@ -112,15 +114,24 @@ int build_tiles(B *b) {
     for (int s = 0; s < L.geo.V; s += b->block) tiles.push_back(int2{(int)l, s});
     L.ntiles = (int)tiles.size() - L.tile0;
   }
+  // within each part the lattices with more steps first, so that the tiles
+  // still running at tick t are a prefix of the part's tiles
+  std::vector<size_t> order(b->lat.size());
+  for (size_t l = 0; l < order.size(); ++l) order[l] = l;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return b->lat[x].npairs > b->lat[y].npairs; });
+  b->active.assign((size_t)3 * (b->maxpairs + 1), 0);
   for (int pass : {4, 3, 0}) {
     const int before = (int)ltiles.size();
-    for (size_t l = 0; l < b->lat.size(); ++l) {
+    const int part = pass == 4 ? 0 : pass == 3 ? 1 : 2;
+    for (size_t l : order) {
       auto &L = b->lat[l];
       if (L.spec != pass) continue;
       L.ltile0 = (int)ltiles.size();
       for (int s = 0; s < L.geo.V; s += b->block)
         for (int mu = 0; mu < L.geo.nd; ++mu) ltiles.push_back(int2{(int)l, 4 * s + mu});
       L.nltiles = (int)ltiles.size() - L.ltile0;
+      for (int t = 0; t <= L.npairs && t <= b->maxpairs; ++t) b->active[3 * t + part] += L.nltiles;
     }
     if (pass == 4) b->nspec4 = (int)ltiles.size() - before;
     if (pass == 3) b->nspec3 = (int)ltiles.size() - before;
@@ -217,7 +228,8 @@ int enqueue(B *b, bool devmom, Rec *rec = nullptr) {
       for (int part = 0; part < 3; ++part)
         su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
                                 b->ar, t, flags, b->partials, b->grid, b->block,
-                                part == 0 ? b->stream : b->pstream[part - 1], part);
+                                part == 0 ? b->stream : b->pstream[part - 1], part,
+                                &b->active[3 * t]);
     for (int k = 0; k < 2; ++k) {
       BH(hipEventRecord(b->ev_join[k], b->pstream[k]));
       BH(hipStreamWaitEvent(b->stream, b->ev_join[k], 0));
@@ -226,13 +238,15 @@ int enqueue(B *b, bool devmom, Rec *rec = nullptr) {
   for (int t = 0; t <= b->maxpairs && (rec || !b->concurrent); ++t) {
     if (!rec) {
       su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
-                              b->ar, t, flags, b->partials, b->grid, b->block, b->stream);
+                              b->ar, t, flags, b->partials, b->grid, b->block, b->stream, -1,
+                              &b->active[3 * t]);
       continue;
     }
     for (int part = 0; part < 3; ++part) {
       beg(RK_TICK4 + part);
       su2k::launch_batch_tick(b->prec, b->dlat, b->dltiles, b->nltiles, b->nspec4, b->nspec3,
-                              b->ar, t, flags, b->partials, b->grid, b->block, b->stream, part);
+                              b->ar, t, flags, b->partials, b->grid, b->block, b->stream, part,
+                              &b->active[3 * t]);
       fin();
     }
   }
@@ -544,8 +558,9 @@ int su2gpu_batch_use_graph(su2gpu_batch *b, int on) {
 // plaquette sums, the MD ticks of the SU(2) d = 4 part, of the d = 3 part, of
 // the remaining (generic) part, the final restore and the reduction;
 // out[8 .. 10]: link updates of the three parts; out[11 .. 13]: their active
-// tile fraction (tile-ticks of lattices not yet finished over all tile-ticks
-// launched: lattices with fewer steps idle in the later ticks).
+// tile fraction (tile-ticks of lattices not yet finished over the part's
+// tiles times its ticks: what the step-count spread leaves of a full launch
+// per tick; the launches cover only the active tiles).
 int su2gpu_batch_profile(su2gpu_batch *b, uint64_t traj, double *out, size_t nout) {
   if (!b || !out) return bfail(SU2GPU_EINVAL, "null pointer");
   if (nout < 14) return bfail(SU2GPU_EINVAL, "batch profile: out needs 14 entries");

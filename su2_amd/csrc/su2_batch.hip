@@ -313,10 +313,11 @@ __global__ __launch_bounds__(256) void k_batch_reduce(const BLat *__restrict__ l
 // ------------------------------------------------------------- launchers ----
 void launch_batch_tick(int prec, const BLat *lats, const int2 *tiles, int ntiles, int n4, int n3,
                        const BArenas &ar, int tick, int flags, double *partials, int grid,
-                       int block, hipStream_t st, int only) {
+                       int block, hipStream_t st, int only, const int *active) {
   const int parts[3][2] = {{0, n4}, {n4, n3}, {n4 + n3, ntiles - n4 - n3}};
   for (int k = 0; k < 3; ++k) {
-    const int t0 = parts[k][0], n = parts[k][1];
+    const int t0 = parts[k][0];
+    const int n = active ? (active[k] < parts[k][1] ? active[k] : parts[k][1]) : parts[k][1];
     if (n <= 0 || (only >= 0 && k != only)) continue;
     const dim3 g(grid < n ? grid : n), bl(block);
 #define SU2_TICK(T, S) \

@@ -35,10 +35,13 @@ struct BArenas {
 // lattices in d = 4 ([0, n4)), then in d = 3 ([n4, n4 + n3)), then the rest;
 // partial rows are nltiles long.  One launch per part (at most `grid` blocks
 // each), so that each part has its own register budget; only >= 0 launches
-// that part alone (0: d = 4, 1: d = 3, 2: the rest).
+// that part alone (0: d = 4, 1: d = 3, 2: the rest).  active (optional): the
+// number of leading tiles of each part that belong to lattices still running
+// at this tick (each part's lattices are ordered by step count, longest
+// first); the launch covers only those.
 void launch_batch_tick(int prec, const BLat *lats, const int2 *ltiles, int nltiles, int n4, int n3,
                        const BArenas &ar, int tick, int flags, double *partials, int grid,
-                       int block, hipStream_t st, int only = -1);
+                       int block, hipStream_t st, int only = -1, const int *active = nullptr);
 void launch_batch_plaq(int prec, const BLat *lats, const int2 *tiles, int ntiles, const BArenas &ar,
                        int use_final, int row, int rowlen, double *partials, int grid, int block,
                        hipStream_t st);
