@@ -445,8 +445,9 @@ def kernel_name(prec, L):
     it: <T, WRITE_U, ACCUM, SPEC, ZS, AN, NT, link in, link out>.  ZS: 1 the
     +-z lane exchange within a wave (64 % L == 0), 2 the general exchange
     (FP32 beyond 256 MiB otherwise), 0 none.  NT: non-temporal stores (1)
-    and momentum loads (2: the FP32 general-exchange step, the FP64 step
-    beyond 1 GiB) where the two link fields and the momenta exceed 256 MiB.
+    and momentum loads (2: the FP32 general-exchange step, both forms with the
+    lane exchange or in FP64 beyond 1 GiB) where the two link fields and the
+    momenta exceed 256 MiB.
     The FP64 form steps on the half-field layout (QHalf in, QHalfW out)."""
     r = 8 if prec == 0 else 4
     big = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
@@ -455,7 +456,8 @@ def kernel_name(prec, L):
         nt = (2 if 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3 else 1) if big else 0
         return f"k_md_su2_4d<double, true, false, 1, {zs}, false, {nt}, su2k::QHalf, su2k::QHalfW>"
     zs = 1 if 64 % L == 0 else (2 if big else 0)
-    nt = (2 if zs == 2 else 1) if big else 0
+    huge = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3
+    nt = (2 if zs == 2 or (zs == 1 and huge) else 1) if big else 0
     return (f"k_md_su2_4d<float, true, false, 1, {zs}, false, {nt}, su2dev::quat<float> const*, "
             "su2dev::quat<float>*>")
 
