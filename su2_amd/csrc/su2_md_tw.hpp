@@ -1,0 +1,214 @@
+// su2_md_tw.hpp -- the specialised SU(2) d = 4 "P then U" step walking t
+// (get_staples.hh:26-80, flat-gaugemonomial.hh:105-131, update_momenta.hh:27-30,
+// update_gauge.hh:11-14): the arithmetic of md_su2_core (su2_md_fast.hpp), bit
+// for bit, with fewer link loads.
+//
+// A thread keeps one 3-volume site and one direction mu and walks the t-slices
+// of its XCD's chunk [xcd Tc, (xcd + 1) Tc) upwards.  The staple links that the
+// link at t + 1 shares with the link at t stay in registers:
+//   * mu != t, plane t: U_t(s+mu-t), U_mu(s-t), U_t(s-t) of the link at t + 1
+//     are U_t(s+mu), U_mu(s), U_t(s) of the link at t, and its own link
+//     U_mu(s+t) is the link at t's U_mu(s+t): 2 loads instead of 6 + 1;
+//   * mu = t, planes nu: U_nu(s) and U_nu(s-nu) of the link at t + 1 are
+//     U_nu(s+t) and U_nu(s+t-nu) of the link at t: 4 loads instead of 6.
+// With the +-z lane exchange (ZS = 1) that leaves 10-11 link loads per link
+// instead of 14-15; the chunk's first slice loads what the slice below would
+// have passed on (its t-1 neighbours come from the neighbouring XCD's chunk).
+//
+// Launch: 32 nb3 blocks of NTF threads (nb3 = vol3 / NTF); hardware block b
+// runs on XCD b % 8, which owns the chunk; local = b / 8 = 4 q + mu, site
+// o3 = q NTF + tid.  Needs the t-blocked layout (Lt % 8 == 0).
+#pragma once
+#include "su2_md_fast.hpp"
+
+namespace su2k {
+
+struct TwSite {
+  int mu, o3, t0;
+  int up[4], dn[4];  // x, y, z neighbour offsets of o3 in [1..3]; [0] per slice
+};
+
+__host__ __device__ __forceinline__ TwSite tw_site(const FastGeo &g, uint32_t b, uint32_t tid) {
+  TwSite f;
+  const uint32_t xcd = b & 7u, local = b >> 3;
+  f.mu = (int)(local & 3u);
+  const uint32_t o3 = (local >> 2) * (uint32_t)NTF + tid;
+  f.o3 = (int)o3;
+  f.t0 = (int)xcd * g.Tc;
+  const uint32_t x = fast_div(o3, g.m_plane);
+  const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
+  const uint32_t y = fast_div(rem3, g.m_Lz);
+  const uint32_t z = rem3 - y * (uint32_t)g.Lz;
+  f.up[0] = f.dn[0] = 0;
+  f.up[1] = ((int)x == g.Lx - 1) ? -(g.Lx - 1) * g.plane : g.plane;
+  f.dn[1] = (x == 0) ? (g.Lx - 1) * g.plane : -g.plane;
+  f.up[2] = ((int)y == g.Ly - 1) ? -(g.Ly - 1) * g.Lz : g.Lz;
+  f.dn[2] = (y == 0) ? (g.Ly - 1) * g.Lz : -g.Lz;
+  f.up[3] = ((int)z == g.Lz - 1) ? -(g.Lz - 1) : 1;
+  f.dn[3] = (z == 0) ? (g.Lz - 1) : -1;
+  return f;
+}
+
+// periodic t-neighbour offsets of slice t
+__host__ __device__ __forceinline__ void tw_t_offsets(const FastGeo &g, int t, int &up0,
+                                                      int &dn0) {
+  up0 = (t == g.Lt - 1) ? -(g.Lt - 1) * g.vol3 : g.vol3;
+  dn0 = (t == 0) ? (g.Lt - 1) * g.vol3 : -g.vol3;
+}
+
+// blocks of a t-walk launch
+inline uint32_t tw_blocks(const FastGeo &g) { return 32u * (uint32_t)g.nb3; }
+
+template <typename T, int MU, int ZS, int NT, class UI, class UO>
+__device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ P,
+                                               const FastGeo &g, const FusedCoef &c,
+                                               const TwSite &f) {
+  static_assert(ZS == 0 || ZS == 1, "t-walk: lane exchange within whole z-rows or none");
+  const int V = g.V;
+  int up[4] = {0, f.up[1], f.up[2], f.up[3]}, dn[4] = {0, f.dn[1], f.dn[2], f.dn[3]};
+  const UI Um = fld_add(Uin, (size_t)MU * V);
+  const UI Ut = Uin;  // direction 0 (t)
+  int zup = 0, zdn = 0;
+  if (ZS == 1) {
+    const int lane = threadIdx.x & 63, zl = lane & (g.Lz - 1);
+    zup = (zl == g.Lz - 1) ? lane - (g.Lz - 1) : lane + 1;
+    zdn = (zl == 0) ? lane + (g.Lz - 1) : lane - 1;
+  }
+  int s = f.t0 * g.vol3 + f.o3;
+  tw_t_offsets(g, f.t0, up[0], dn[0]);
+  // carried links
+  quat<T> u;            // own link U_mu(s)
+  quat<T> D0, E0, F0;   // mu != t, plane t: U_t(s+mu-t), U_mu(s-t), U_t(s-t)
+  quat<T> Cc[3], Fc[3]; // mu = t, plane nu = k + 1: U_nu(s), U_nu(s-nu)
+  if constexpr (MU != 0) {
+    u = ldq(Um, (uint32_t)s);
+    E0 = ldq(Um, (uint32_t)(s + dn[0]));
+    F0 = ldq(Ut, (uint32_t)(s + dn[0]));
+    if (ZS == 1 && MU == 3) D0 = qshfl(F0, zup);
+    else D0 = ldq(Ut, (uint32_t)(s + up[MU] + dn[0]));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int nu = k + 1;
+      const UI Un = fld_add(Uin, (size_t)nu * V);
+      Cc[k] = ldq(Un, (uint32_t)s);
+      if (ZS == 1 && nu == 3) Fc[k] = qshfl(Cc[k], zdn);
+      else Fc[k] = ldq(Un, (uint32_t)(s + dn[nu]));
+    }
+  }
+  for (int kt = 0; kt < g.Tc; ++kt) {
+    if (kt) {
+      s += g.vol3;
+      tw_t_offsets(g, f.t0 + kt, up[0], dn[0]);
+    }
+    quat<T> A[3], B[3], C[3], D[3], E[3], F[3];
+    if constexpr (MU != 0) {
+      // plane t (k = 0)
+      B[0] = ldq(Um, (uint32_t)(s + up[0]));  // U_mu(s+t): the next slice's own link
+      C[0] = ldq(Ut, (uint32_t)s);            // U_t(s)
+      if (!(ZS == 1 && MU == 3)) A[0] = ldq(Ut, (uint32_t)(s + up[MU]));  // U_t(s+mu)
+      D[0] = D0;
+      E[0] = E0;
+      F[0] = F0;
+      // planes nu != t, mu: the loads of md_su2_core
+#pragma unroll
+      for (int k = 1; k < 3; ++k) {
+        const int nu = k + (k >= MU ? 1 : 0);
+        const UI Un = fld_add(Uin, (size_t)nu * V);
+        if (ZS && nu == 3) {
+          A[k] = ldq(Un, (uint32_t)(s + up[MU]));  // U_z(s+mu)
+          C[k] = ldq(Un, (uint32_t)s);             // U_z(s)
+        } else if (ZS && MU == 3) {
+          B[k] = ldq(Um, (uint32_t)(s + up[nu]));  // U_z(s+nu)
+          C[k] = ldq(Un, (uint32_t)s);             // U_nu(s)
+          E[k] = ldq(Um, (uint32_t)(s + dn[nu]));  // U_z(s-nu)
+          F[k] = ldq(Un, (uint32_t)(s + dn[nu]));  // U_nu(s-nu)
+        } else {
+          A[k] = ldq(Un, (uint32_t)(s + up[MU]));
+          B[k] = ldq(Um, (uint32_t)(s + up[nu]));
+          C[k] = ldq(Un, (uint32_t)s);
+          D[k] = ldq(Un, (uint32_t)(s + up[MU] + dn[nu]));
+          E[k] = ldq(Um, (uint32_t)(s + dn[nu]));
+          F[k] = ldq(Un, (uint32_t)(s + dn[nu]));
+        }
+      }
+      if (ZS == 1 && MU == 3) A[0] = qshfl(C[0], zup);  // U_t(s+z)
+      if (ZS) {
+#pragma unroll
+        for (int k = 1; k < 3; ++k) {
+          const int nu = k + (k >= MU ? 1 : 0);
+          if (nu == 3) {
+            B[k] = qshfl(u, zup);     // U_mu(s+z)
+            E[k] = qshfl(u, zdn);     // U_mu(s-z)
+            D[k] = qshfl(A[k], zdn);  // U_z(s+mu-z)
+            F[k] = qshfl(C[k], zdn);  // U_z(s-z)
+          } else if (MU == 3) {
+            A[k] = qshfl(C[k], zup);  // U_nu(s+z)
+            D[k] = qshfl(F[k], zup);  // U_nu(s+z-nu)
+          }
+        }
+      }
+    } else {
+      u = ldq(Um, (uint32_t)s);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int nu = k + 1;
+        const UI Un = fld_add(Uin, (size_t)nu * V);
+        C[k] = Cc[k];  // U_nu(s)
+        F[k] = Fc[k];  // U_nu(s-nu)
+        A[k] = ldq(Un, (uint32_t)(s + up[0]));  // U_nu(s+t)
+        if (!(ZS && nu == 3)) {
+          B[k] = ldq(Um, (uint32_t)(s + up[nu]));          // U_t(s+nu)
+          D[k] = ldq(Un, (uint32_t)(s + up[0] + dn[nu]));  // U_nu(s+t-nu)
+          E[k] = ldq(Um, (uint32_t)(s + dn[nu]));          // U_t(s-nu)
+        }
+      }
+      if (ZS) {
+        B[2] = qshfl(u, zup);     // U_t(s+z)
+        E[2] = qshfl(u, zdn);     // U_t(s-z)
+        D[2] = qshfl(A[2], zdn);  // U_z(s+t-z)
+      }
+    }
+    // -0.0 is the identity of FP64 addition: the first staple's addition folds away
+    double K[4] = {-0.0, -0.0, -0.0, -0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      staple_pair<T, false>(K, 1.0, A[k], B[k], C[k], D[k], E[k], F[k]);
+    T *__restrict__ P0 = P + (size_t)(MU * 3 + 0) * V;
+    T *__restrict__ P1 = P + (size_t)(MU * 3 + 1) * V;
+    T *__restrict__ P2 = P + (size_t)(MU * 3 + 2) * V;
+    const T pi0 = ldm<NT == 2>(P0, (uint32_t)s), pi1 = ldm<NT == 2>(P1, (uint32_t)s),
+            pi2 = ldm<NT == 2>(P2, (uint32_t)s);
+    T pn[3];
+    (void)md_su2_update<T, true, NT, UO>(u, K, pi0, pi1, pi2, P0, P1, P2,
+                                         fld_add(Uout, (size_t)MU * V), (uint32_t)s, c, pn);
+    // what the next slice's link shares with this one
+    if constexpr (MU != 0) {
+      E0 = u;
+      u = B[0];
+      D0 = A[0];
+      F0 = C[0];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        Cc[k] = A[k];
+        Fc[k] = D[k];
+      }
+    }
+  }
+}
+
+// ZS: 1 the +-z lane exchange (64 % Lz == 0), 0 none; NT as k_md_su2_4d
+template <typename T, int ZS, int NT, class UI = const quat<T> *, class UO = quat<T> *>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_tw(UI Uin, UO Uout, T *__restrict__ P,
+                                                    FastGeo g, FusedCoef c) {
+  const TwSite f = tw_site(g, blockIdx.x, threadIdx.x);
+  switch (__builtin_amdgcn_readfirstlane(f.mu)) {
+    case 0: md_su2_tw_body<T, 0, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    default: md_su2_tw_body<T, 3, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+  }
+}
+
+}  // namespace su2k

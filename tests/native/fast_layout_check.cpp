@@ -6,7 +6,7 @@
 #include <cstdio>
 #include <vector>
 
-#include "../../su2_amd/csrc/su2_md_fast.hpp"
+#include "../../su2_amd/csrc/su2_md_tw.hpp"
 
 using namespace su2k;
 
@@ -59,6 +59,52 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
       printf("FAIL link %zu not visited\n", i);
       return 1;
     }
+  // the t-walk shape (t-blocked layouts): 32 nb3 blocks, each thread one
+  // (3-volume site, mu) walking the Tc slices of its XCD's chunk
+  if (g.tblock && !wmu) {
+    std::vector<unsigned char> seen_t((size_t)4 * V, 0);
+    const uint32_t nbt = tw_blocks(g);
+    for (uint32_t b = 0; b < nbt; ++b)
+      for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
+        const TwSite f = tw_site(g, b, tid);
+        if (f.mu < 0 || f.mu > 3 || f.o3 < 0 || f.o3 >= g.vol3 || f.t0 < 0 || f.t0 + g.Tc > L[0]) {
+          printf("FAIL t-walk range b=%u tid=%u mu=%d o3=%d t0=%d\n", b, tid, f.mu, f.o3, f.t0);
+          return 1;
+        }
+        for (int kt = 0; kt < g.Tc; ++kt) {
+          const int t = f.t0 + kt, s = t * g.vol3 + f.o3;
+          if (seen_t[(size_t)f.mu * V + s]++) {
+            printf("FAIL t-walk link (s=%d, mu=%d) visited twice\n", s, f.mu);
+            return 1;
+          }
+          int up[4], dn[4];
+          tw_t_offsets(g, t, up[0], dn[0]);
+          for (int d = 1; d < 4; ++d) {
+            up[d] = f.up[d];
+            dn[d] = f.dn[d];
+          }
+          int c[4], rest = s;
+          for (int d = 0; d < 4; ++d) {
+            c[d] = rest / str[d];
+            rest -= c[d] * str[d];
+          }
+          for (int d = 0; d < 4; ++d) {
+            const int upn = s + ((c[d] + 1) % L[d] - c[d]) * str[d];
+            const int dnn = s + ((c[d] + L[d] - 1) % L[d] - c[d]) * str[d];
+            if (s + up[d] != upn || s + dn[d] != dnn) {
+              printf("FAIL t-walk neighbours of s=%d along %d\n", s, d);
+              return 1;
+            }
+          }
+        }
+      }
+    for (size_t i = 0; i < seen_t.size(); ++i)
+      if (seen_t[i] != 1) {
+        printf("FAIL t-walk link %zu not visited\n", i);
+        return 1;
+      }
+    printf("%dx%dx%dx%d t-walk: ok (%u blocks)\n", L[0], L[1], L[2], L[3], nbt);
+  }
   // the thread-per-site shape: V / NTF blocks, every site exactly once
   const uint32_t nbs = (uint32_t)(V / NTF);
   if (g.tblock && nbs % 8 != 0) {
@@ -96,7 +142,7 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
 
 int main() {
   // extents (t, x, y, z)
-  const int geoms[][4] = {{8, 8, 8, 8},   {16, 16, 16, 16}, {32, 32, 32, 32}, {24, 24, 24, 24},
+  const int geoms[][4] = {{32, 8, 8, 8},  {8, 8, 8, 8},     {16, 16, 16, 16}, {32, 32, 32, 32}, {24, 24, 24, 24},
                           {48, 48, 48, 48}, {16, 8, 8, 8},  {12, 8, 8, 8},    {4, 8, 8, 8},
                           {8, 4, 8, 16},  {8, 16, 16, 1},   {40, 16, 4, 4},   {8, 12, 12, 12},
                           {64, 16, 16, 16}};

@@ -696,6 +696,49 @@ print("NT_CHILD_OK", flush=True)
 """
 
 
+_TW_CHILD = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import su2_amd as S
+from oracle.oracle import Oracle
+orc = Oracle()
+dims = (8, 8, 8, 32, 4)
+U = orc.hotstart(1, dims, 91, 0.5)
+P = orc.initnormal(1, dims, 92)
+with S.Lattice(S.SU2, dims, 2.4, precision=S.FP32) as lat:
+    lat.set_links(U)
+    r = lat.md_trajectory(P, 6, 6)
+    lat.md_finish(True)
+    np.save(sys.argv[2], lat.get_links())
+    print("TW_CHILD_OK", repr(r["deltaH"]), flush=True)
+"""
+
+
+def test_fp32_t_walk_step_is_bit_identical(tmp_path):
+    """The FP32 P-then-U step as a t-walk (k_md_su2_4d_tw: a thread walks the
+    t-slices of its XCD's chunk, keeping the staple links that neighbouring
+    slices share; on by default for FP32 lane-exchange lattices within the
+    Infinity Cache with >= 4 slices per XCD) against the one-launch-per-link
+    step (SU2GPU_TW=0): one OMF2(6) trajectory on (Lx, Ly, Lz, Lt) = (8, 8, 8,
+    32), in two child processes because the switch is read once per process;
+    links and dH bit for bit."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = {}
+    for tw in ("1", "0"):
+        path = tmp_path / f"links_tw{tw}.npy"
+        e = dict(os.environ, SU2GPU_TW=tw)
+        out = subprocess.run([sys.executable, "-c", _TW_CHILD, root, str(path)], env=e,
+                             capture_output=True, text=True, timeout=300)
+        print(out.stdout)
+        assert out.returncode == 0, out.stdout + out.stderr
+        line = [ln for ln in out.stdout.splitlines() if ln.startswith("TW_CHILD_OK")][0]
+        outs[tw] = (np.load(path), line.split()[1])
+    np.testing.assert_array_equal(outs["1"][0], outs["0"][0])
+    assert outs["1"][1] == outs["0"][1]
+
+
 @pytest.mark.parametrize("env", [{"SU2GPU_NT": "1", "SU2GPU_XWALK": "1"},
                                  {"SU2GPU_NT": "1", "SU2GPU_XWALK": "0"},
                                  {"SU2GPU_NT": "2"},
