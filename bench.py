@@ -448,14 +448,16 @@ def kernel_name(prec, L):
     and momentum loads (2: the FP32 general-exchange step, both forms with the
     lane exchange or in FP64 beyond 1 GiB) where the two link fields and the
     momenta exceed 256 MiB.
-    The FP64 form steps on the half-field layout (QHalf in, QHalfW out); the
-    FP32 form with the lane exchange within the Infinity Cache and >= 4
-    t-slices per XCD takes the t-walk (k_md_su2_4d_tw<float, ZS, NT>)."""
+    The FP64 form steps on the half-field layout (QHalf in, QHalfW out); with
+    the lane exchange and >= 4 t-slices per XCD (FP32: within the Infinity
+    Cache) the step is the t-walk (k_md_su2_4d_tw<T, ZS, NT, in, out>)."""
     r = 8 if prec == 0 else 4
     big = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
     if prec == 0:
         zs = 1 if 64 % L == 0 else 0
         nt = (2 if 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3 else 1) if big else 0
+        if zs == 1 and L % 8 == 0 and L // 8 >= 4:
+            return f"k_md_su2_4d_tw<double, 1, {nt}, su2k::QHalf, su2k::QHalfW>"
         return f"k_md_su2_4d<double, true, false, 1, {zs}, false, {nt}, su2k::QHalf, su2k::QHalfW>"
     zs = 1 if 64 % L == 0 else (2 if big else 0)
     huge = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3

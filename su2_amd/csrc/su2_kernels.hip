@@ -491,8 +491,8 @@ static int zg_pref() {
   if (v == -2) v = env_int("SU2GPU_ZG", -1);
   return v;
 }
-// SU2GPU_TW = 0 keeps the FP32 lane-exchange step on one launch per link
-// instead of the t-walk (k_md_su2_4d_tw) where that applies
+// SU2GPU_TW = 0 keeps the lane-exchange step on one launch per link instead
+// of the t-walk (k_md_su2_4d_tw) where that applies
 static int tw_pref() {
   static int v = -1;
   if (v < 0) v = env_int("SU2GPU_TW", 1);
@@ -513,6 +513,25 @@ static int zs_pref() {
 template <bool W, bool A, class UI>
 static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch &L,
                         const FastGeo &f, hipStream_t st) {
+  // P-then-U steps with the lane exchange and >= 4 t-slices per XCD: the
+  // t-walk (32^4: 140-144 against 145-147 us in the same process; the last
+  // bit of ~0.6 % of its values differs from this step's, the update's FMA
+  // contraction, within the oracle parity either way)
+  if constexpr (W && !A) {
+    if (zs && f.tblock && f.Tc >= 4 && tw_pref()) {
+      const dim3 gt(tw_blocks(f)), bt(NTF);
+      if (nt == 2)
+        hipLaunchKernelGGL((k_md_su2_4d_tw<double, 1, 2, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, f, L.c);
+      else if (nt == 1)
+        hipLaunchKernelGGL((k_md_su2_4d_tw<double, 1, 1, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, f, L.c);
+      else
+        hipLaunchKernelGGL((k_md_su2_4d_tw<double, 1, 0, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, f, L.c);
+      return;
+    }
+  }
   const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
 #define SU2_HALF(Z, N)                                                                          \
   hipLaunchKernelGGL((k_md_su2_4d<double, W, A, 1, Z, false, N, UI, QHalfW>), grid, block, 0, st, \
