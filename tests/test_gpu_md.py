@@ -678,8 +678,8 @@ from oracle.oracle import Oracle
 orc = Oracle()
 # (Lx, Ly, Lz, Lt) = (8, 16, 12, 8): Lz = 12 does not divide 64, the FP32
 # step's general z exchange (neighbours in another wave loaded by the lanes
-# that need them)
-for dims in [(8, 8, 8, 8, 4), (2, 64, 64, 8, 4), (8, 16, 12, 8, 4)]:
+# that need them); (8, 8, 8, 32): 4 t-slices per XCD, the t-walk step
+for dims in [(8, 8, 8, 8, 4), (2, 64, 64, 8, 4), (8, 16, 12, 8, 4), (8, 8, 8, 32, 4)]:
     U = orc.hotstart(1, dims, 77, 0.3)
     P = orc.initnormal(1, dims, 78)
     Uw, _ = orc.integrate(1, dims, U, P, 2.4, 6, 5)
