@@ -465,6 +465,9 @@ def kernel_name(prec, L):
     if zs == 1 and not big and L % 8 == 0 and L // 8 >= 4:
         # the t-walk: each thread walks the t-slices of its XCD's chunk
         return "k_md_su2_4d_tw<float, 1, 0, su2dev::quat<float> const*, su2dev::quat<float>*>"
+    if zs == 2 and L % 8 == 0 and L // 8 >= 4:
+        # the t-walk with the general exchange, one direction per wave
+        return "k_md_su2_4d_twz<float, 2, su2dev::quat<float> const*, su2dev::quat<float>*>"
     return (f"k_md_su2_4d<float, true, false, 1, {zs}, false, {nt}, su2dev::quat<float> const*, "
             "su2dev::quat<float>*>")
 

@@ -18,6 +18,13 @@
 // Launch: 32 nb3 blocks of NTF threads (nb3 = vol3 / NTF); hardware block b
 // runs on XCD b % 8, which owns the chunk; local = b / 8 = 4 q + mu, site
 // o3 = q NTF + tid.  Needs the t-blocked layout (Lt % 8 == 0).
+//
+// k_md_su2_4d_twz: the same walk with the general +-z exchange (ZS = 2, any
+// Lz: the neighbour of lane l is lane l +- 1, or l -+ (Lz - 1) at a row's wrap,
+// where that lane is in the wave; the lanes whose neighbour is in another
+// wave load it in the same memory round trip) and one direction per wave: a
+// block holds 64 sites (o3 = q 64 + lane) and its wave w the direction w, so
+// the four directions of the same sites walk t together on one CU.
 #pragma once
 #include "su2_md_fast.hpp"
 
@@ -49,6 +56,29 @@ __host__ __device__ __forceinline__ TwSite tw_site(const FastGeo &g, uint32_t b,
   return f;
 }
 
+// the mapping of k_md_su2_4d_twz: local = b / 8 = q, mu = wave, o3 = q 64 + lane
+__host__ __device__ __forceinline__ TwSite tw_site_w(const FastGeo &g, uint32_t b, uint32_t tid) {
+  TwSite f;
+  const uint32_t xcd = b & 7u, q = b >> 3;
+  f.mu = (int)(tid >> 6);
+  const uint32_t o3 = q * 64u + (tid & 63u);
+  f.o3 = (int)o3;
+  f.t0 = (int)xcd * g.Tc;
+  const uint32_t x = fast_div(o3, g.m_plane);
+  const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
+  const uint32_t y = fast_div(rem3, g.m_Lz);
+  const uint32_t z = rem3 - y * (uint32_t)g.Lz;
+  f.up[0] = f.dn[0] = 0;
+  f.up[1] = ((int)x == g.Lx - 1) ? -(g.Lx - 1) * g.plane : g.plane;
+  f.dn[1] = (x == 0) ? (g.Lx - 1) * g.plane : -g.plane;
+  f.up[2] = ((int)y == g.Ly - 1) ? -(g.Ly - 1) * g.Lz : g.Lz;
+  f.dn[2] = (y == 0) ? (g.Ly - 1) * g.Lz : -g.Lz;
+  f.up[3] = ((int)z == g.Lz - 1) ? -(g.Lz - 1) : 1;
+  f.dn[3] = (z == 0) ? (g.Lz - 1) : -1;
+  return f;
+}
+inline uint32_t tw_blocks_w(const FastGeo &g) { return 8u * (uint32_t)(g.vol3 / 64); }
+
 // periodic t-neighbour offsets of slice t
 __host__ __device__ __forceinline__ void tw_t_offsets(const FastGeo &g, int t, int &up0,
                                                       int &dn0) {
@@ -63,16 +93,25 @@ template <typename T, int MU, int ZS, int NT, class UI, class UO>
 __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ P,
                                                const FastGeo &g, const FusedCoef &c,
                                                const TwSite &f) {
-  static_assert(ZS == 0 || ZS == 1, "t-walk: lane exchange within whole z-rows or none");
+  static_assert(ZS >= 0 && ZS <= 2, "t-walk: ZS 0, 1 or 2");
   const int V = g.V;
   int up[4] = {0, f.up[1], f.up[2], f.up[3]}, dn[4] = {0, f.dn[1], f.dn[2], f.dn[3]};
   const UI Um = fld_add(Uin, (size_t)MU * V);
   const UI Ut = Uin;  // direction 0 (t)
   int zup = 0, zdn = 0;
+  bool vup = true, vdn = true;  // ZS = 2: the neighbour lane is in the wave
   if (ZS == 1) {
     const int lane = threadIdx.x & 63, zl = lane & (g.Lz - 1);
     zup = (zl == g.Lz - 1) ? lane - (g.Lz - 1) : lane + 1;
     zdn = (zl == 0) ? lane + (g.Lz - 1) : lane - 1;
+  } else if (ZS == 2) {
+    const int lane = threadIdx.x & 63;
+    zup = lane + up[3];
+    zdn = lane + dn[3];
+    vup = (unsigned)zup < 64u;
+    vdn = (unsigned)zdn < 64u;
+    if (!vup) zup = lane;
+    if (!vdn) zdn = lane;
   }
   int s = f.t0 * g.vol3 + f.o3;
   tw_t_offsets(g, f.t0, up[0], dn[0]);
@@ -84,16 +123,26 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
     u = ldq(Um, (uint32_t)s);
     E0 = ldq(Um, (uint32_t)(s + dn[0]));
     F0 = ldq(Ut, (uint32_t)(s + dn[0]));
-    if (ZS == 1 && MU == 3) D0 = qshfl(F0, zup);
-    else D0 = ldq(Ut, (uint32_t)(s + up[MU] + dn[0]));
+    if (ZS && MU == 3) {
+      if (ZS == 2 && !vup) D0 = ldq(Ut, (uint32_t)(s + up[3] + dn[0]));
+      const quat<T> d = qshfl(F0, zup);  // U_t(s+z-t)
+      if (ZS == 1 || vup) D0 = d;
+    } else {
+      D0 = ldq(Ut, (uint32_t)(s + up[MU] + dn[0]));
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int nu = k + 1;
       const UI Un = fld_add(Uin, (size_t)nu * V);
       Cc[k] = ldq(Un, (uint32_t)s);
-      if (ZS == 1 && nu == 3) Fc[k] = qshfl(Cc[k], zdn);
-      else Fc[k] = ldq(Un, (uint32_t)(s + dn[nu]));
+      if (ZS && nu == 3) {
+        if (ZS == 2 && !vdn) Fc[k] = ldq(Un, (uint32_t)(s + dn[3]));
+        const quat<T> f = qshfl(Cc[k], zdn);  // U_z(s-z)
+        if (ZS == 1 || vdn) Fc[k] = f;
+      } else {
+        Fc[k] = ldq(Un, (uint32_t)(s + dn[nu]));
+      }
     }
   }
   for (int kt = 0; kt < g.Tc; ++kt) {
@@ -106,7 +155,8 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
       // plane t (k = 0)
       B[0] = ldq(Um, (uint32_t)(s + up[0]));  // U_mu(s+t): the next slice's own link
       C[0] = ldq(Ut, (uint32_t)s);            // U_t(s)
-      if (!(ZS == 1 && MU == 3)) A[0] = ldq(Ut, (uint32_t)(s + up[MU]));  // U_t(s+mu)
+      if (!(ZS && MU == 3)) A[0] = ldq(Ut, (uint32_t)(s + up[MU]));  // U_t(s+mu)
+      else if (ZS == 2 && !vup) A[0] = ldq(Ut, (uint32_t)(s + up[3]));
       D[0] = D0;
       E[0] = E0;
       F[0] = F0;
@@ -118,11 +168,24 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
         if (ZS && nu == 3) {
           A[k] = ldq(Un, (uint32_t)(s + up[MU]));  // U_z(s+mu)
           C[k] = ldq(Un, (uint32_t)s);             // U_z(s)
+          if (ZS == 2) {
+            // neighbours in another wave, loaded by the lanes that need them
+            if (!vup) B[k] = ldq(Um, (uint32_t)(s + up[3]));
+            if (!vdn) {
+              E[k] = ldq(Um, (uint32_t)(s + dn[3]));
+              D[k] = ldq(Un, (uint32_t)(s + up[MU] + dn[3]));
+              F[k] = ldq(Un, (uint32_t)(s + dn[3]));
+            }
+          }
         } else if (ZS && MU == 3) {
           B[k] = ldq(Um, (uint32_t)(s + up[nu]));  // U_z(s+nu)
           C[k] = ldq(Un, (uint32_t)s);             // U_nu(s)
           E[k] = ldq(Um, (uint32_t)(s + dn[nu]));  // U_z(s-nu)
           F[k] = ldq(Un, (uint32_t)(s + dn[nu]));  // U_nu(s-nu)
+          if (ZS == 2 && !vup) {
+            A[k] = ldq(Un, (uint32_t)(s + up[3]));
+            D[k] = ldq(Un, (uint32_t)(s + up[3] + dn[nu]));
+          }
         } else {
           A[k] = ldq(Un, (uint32_t)(s + up[MU]));
           B[k] = ldq(Um, (uint32_t)(s + up[nu]));
@@ -132,19 +195,34 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
           F[k] = ldq(Un, (uint32_t)(s + dn[nu]));
         }
       }
-      if (ZS == 1 && MU == 3) A[0] = qshfl(C[0], zup);  // U_t(s+z)
+      // every lane takes part in each exchange; with ZS = 2 a lane keeps its
+      // loaded value where its neighbour is in another wave
+      if (ZS && MU == 3) {
+        const quat<T> a = qshfl(C[0], zup);  // U_t(s+z)
+        if (ZS == 1 || vup) A[0] = a;
+      }
       if (ZS) {
 #pragma unroll
         for (int k = 1; k < 3; ++k) {
           const int nu = k + (k >= MU ? 1 : 0);
           if (nu == 3) {
-            B[k] = qshfl(u, zup);     // U_mu(s+z)
-            E[k] = qshfl(u, zdn);     // U_mu(s-z)
-            D[k] = qshfl(A[k], zdn);  // U_z(s+mu-z)
-            F[k] = qshfl(C[k], zdn);  // U_z(s-z)
+            const quat<T> b = qshfl(u, zup);     // U_mu(s+z)
+            const quat<T> e = qshfl(u, zdn);     // U_mu(s-z)
+            const quat<T> d = qshfl(A[k], zdn);  // U_z(s+mu-z)
+            const quat<T> f = qshfl(C[k], zdn);  // U_z(s-z)
+            if (ZS == 1 || vup) B[k] = b;
+            if (ZS == 1 || vdn) {
+              E[k] = e;
+              D[k] = d;
+              F[k] = f;
+            }
           } else if (MU == 3) {
-            A[k] = qshfl(C[k], zup);  // U_nu(s+z)
-            D[k] = qshfl(F[k], zup);  // U_nu(s+z-nu)
+            const quat<T> a = qshfl(C[k], zup);  // U_nu(s+z)
+            const quat<T> d = qshfl(F[k], zup);  // U_nu(s+z-nu)
+            if (ZS == 1 || vup) {
+              A[k] = a;
+              D[k] = d;
+            }
           }
         }
       }
@@ -161,12 +239,23 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
           B[k] = ldq(Um, (uint32_t)(s + up[nu]));          // U_t(s+nu)
           D[k] = ldq(Un, (uint32_t)(s + up[0] + dn[nu]));  // U_nu(s+t-nu)
           E[k] = ldq(Um, (uint32_t)(s + dn[nu]));          // U_t(s-nu)
+        } else if (ZS == 2) {
+          if (!vup) B[k] = ldq(Um, (uint32_t)(s + up[3]));
+          if (!vdn) {
+            E[k] = ldq(Um, (uint32_t)(s + dn[3]));
+            D[k] = ldq(Un, (uint32_t)(s + up[0] + dn[3]));
+          }
         }
       }
       if (ZS) {
-        B[2] = qshfl(u, zup);     // U_t(s+z)
-        E[2] = qshfl(u, zdn);     // U_t(s-z)
-        D[2] = qshfl(A[2], zdn);  // U_z(s+t-z)
+        const quat<T> b = qshfl(u, zup);     // U_t(s+z)
+        const quat<T> e = qshfl(u, zdn);     // U_t(s-z)
+        const quat<T> d = qshfl(A[2], zdn);  // U_z(s+t-z)
+        if (ZS == 1 || vup) B[2] = b;
+        if (ZS == 1 || vdn) {
+          E[2] = e;
+          D[2] = d;
+        }
       }
     }
     // -0.0 is the identity of FP64 addition: the first staple's addition folds away
@@ -208,6 +297,19 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_tw(UI Uin, UO Uout, T *__rest
     case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
     case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
     default: md_su2_tw_body<T, 3, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+  }
+}
+
+// the t-walk with the general +-z exchange, one direction per wave
+template <typename T, int NT, class UI = const quat<T> *, class UO = quat<T> *>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_twz(UI Uin, UO Uout, T *__restrict__ P,
+                                                     FastGeo g, FusedCoef c) {
+  const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
+  switch (__builtin_amdgcn_readfirstlane(f.mu)) {
+    case 0: md_su2_tw_body<T, 0, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    case 1: md_su2_tw_body<T, 1, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    case 2: md_su2_tw_body<T, 2, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    default: md_su2_tw_body<T, 3, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
   }
 }
 

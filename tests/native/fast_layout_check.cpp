@@ -61,12 +61,13 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
     }
   // the t-walk shape (t-blocked layouts): 32 nb3 blocks, each thread one
   // (3-volume site, mu) walking the Tc slices of its XCD's chunk
-  if (g.tblock && !wmu) {
+  // (wmu: the mapping of the general-exchange t-walk, one direction per wave)
+  if (g.tblock) {
     std::vector<unsigned char> seen_t((size_t)4 * V, 0);
-    const uint32_t nbt = tw_blocks(g);
+    const uint32_t nbt = wmu ? tw_blocks_w(g) : tw_blocks(g);
     for (uint32_t b = 0; b < nbt; ++b)
       for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
-        const TwSite f = tw_site(g, b, tid);
+        const TwSite f = wmu ? tw_site_w(g, b, tid) : tw_site(g, b, tid);
         if (f.mu < 0 || f.mu > 3 || f.o3 < 0 || f.o3 >= g.vol3 || f.t0 < 0 || f.t0 + g.Tc > L[0]) {
           printf("FAIL t-walk range b=%u tid=%u mu=%d o3=%d t0=%d\n", b, tid, f.mu, f.o3, f.t0);
           return 1;
@@ -103,7 +104,7 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
         printf("FAIL t-walk link %zu not visited\n", i);
         return 1;
       }
-    printf("%dx%dx%dx%d t-walk: ok (%u blocks)\n", L[0], L[1], L[2], L[3], nbt);
+    printf("%dx%dx%dx%d t-walk%s: ok (%u blocks)\n", L[0], L[1], L[2], L[3], wmu ? " wmu" : "", nbt);
   }
   // the thread-per-site shape: V / NTF blocks, every site exactly once
   const uint32_t nbs = (uint32_t)(V / NTF);

@@ -35,6 +35,9 @@ def test_fast_step_layout(tmp_path):
     # and on the 32 x 8^3 lattice of its GPU parity test
     for ext in ("32x32x32x32", "48x48x48x48", "24x24x24x24", "8x8x8x8", "32x8x8x8"):
         assert f"{ext} t-walk: ok" in out.stdout
+    # the general-exchange t-walk (one direction per wave), Lz = 48, 24 and 8
+    for ext in ("48x48x48x48", "24x24x24x24", "32x8x8x8"):
+        assert f"{ext} t-walk wmu: ok" in out.stdout
     # the Lt = 8 layout (Tc = 1) with the truncated divide-by-1 constant goes
     # out of range; the library codes that divisor separately
     assert "8x8x8x8 truncated d=1 multiplier" in out.stdout
