@@ -458,6 +458,8 @@ def kernel_name(prec, L):
         nt = (2 if 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3 else 1) if big else 0
         if zs == 1 and L % 8 == 0 and L // 8 >= 4:
             return f"k_md_su2_4d_tw<double, 1, {nt}, su2k::QHalf, su2k::QHalfW>"
+        if zs == 0 and L % 8 == 0 and L // 8 >= 4:
+            return f"k_md_su2_4d_twz<double, {nt}, su2k::QHalf, su2k::QHalfW>"
         return f"k_md_su2_4d<double, true, false, 1, {zs}, false, {nt}, su2k::QHalf, su2k::QHalfW>"
     zs = 1 if 64 % L == 0 else (2 if big else 0)
     huge = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3

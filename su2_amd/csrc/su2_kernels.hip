@@ -531,6 +531,22 @@ static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch 
                            (double *)L.P, f, L.c);
       return;
     }
+    // without the whole-row exchange (Lz does not divide 64): the t-walk with
+    // the general exchange, one direction per wave (48^4: 776 against 832 us
+    // in the same process; within the last bit of this step, as above)
+    if (!zs && zs_pref() && zg_pref() != 0 && f.tblock && f.Tc >= 4 && tw_pref()) {
+      const dim3 gt(tw_blocks_w(f)), bt(NTF);
+      if (nt == 2)
+        hipLaunchKernelGGL((k_md_su2_4d_twz<double, 2, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, f, L.c);
+      else if (nt == 1)
+        hipLaunchKernelGGL((k_md_su2_4d_twz<double, 1, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, f, L.c);
+      else
+        hipLaunchKernelGGL((k_md_su2_4d_twz<double, 0, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, f, L.c);
+      return;
+    }
   }
   const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
 #define SU2_HALF(Z, N)                                                                          \
