@@ -385,15 +385,18 @@ __global__ __launch_bounds__(NT) void k_mom_to_aos(const T *__restrict__ src, do
 // Gaussian momenta from the counter-based Philox4x32-10 stream (device
 // throughput mode): link (s, mu) draws from counter (s*nd+mu, 0, traj_lo,
 // traj_hi) with key = seed; Box-Muller on 32-bit uniforms in the field's
-// precision.
+// precision.  Threads run direction-major (thread mu*V + s), so each store
+// instruction of a wave writes one contiguous run of a component plane; the
+// counter, and with it every value, does not depend on that order.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_momenta_philox(T *__restrict__ P, int V, int nd, int nm,
                                                        uint64_t seed, uint64_t traj) {
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i >= V * nd) return;
-  const int s = i / nd, mu = i - s * nd;
+  const int mu = i / V, s = i - mu * V;
   T z[4];
-  philox_normals4<T>((uint32_t)i, (uint32_t)(traj & 0xffffffffu), (uint32_t)(traj >> 32), seed, z);
+  philox_normals4<T>((uint32_t)(s * nd + mu), (uint32_t)(traj & 0xffffffffu),
+                     (uint32_t)(traj >> 32), seed, z);
   for (int k = 0; k < nm; ++k) P[((size_t)mu * nm + k) * V + s] = z[k];
 }
 
@@ -919,6 +922,11 @@ static void launch_plaq_g(const void *Uin, void *Uout, const Geo &g, double w_t,
       break;
   }
 }
+// the plaquette sum of the specialised geometry walks t (k_plaq_su2_4d_tw) on
+// the t-blocked layout (SU2GPU_TW=0: one thread per site and slice), with the
+// +-z lane exchange where 64 % Lz == 0
+static bool plaq_tw_ok(const FastGeo &f) { return f.tblock && tw_pref(); }
+static bool plaq_tw_zs(const FastGeo &f) { return 64 % f.Lz == 0 && zs_pref(); }
 int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, double w_t,
                 double w_s, int mu_start, int flags, double *partials, hipStream_t st,
                 int in_half) {
@@ -931,6 +939,22 @@ int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, 
     const int an = (flags & F_ANISO) ? 1 : 0;
     const size_t nl = 4 * (size_t)g.V;
     const QHalf hin{(const su2_d2 *)Uin, (const su2_d2 *)Uin + nl};
+    if (plaq_tw_ok(f)) {
+      const bool rs = (flags & F_RESTORE) != 0, zx = plaq_tw_zs(f);
+      const dim3 gt(plaq_tw_blocks(f));
+#define SU2_PLAQ_TW(R, Z)                                                                  \
+  hipLaunchKernelGGL((k_plaq_su2_4d_tw<double, R, Z, QHalf>), gt, block, 0, st, hin,       \
+                     (quat<double> *)Uout, f, w_t, w_s, mu_start, an, partials)
+      if (rs) {
+        if (zx) SU2_PLAQ_TW(true, 1);
+        else SU2_PLAQ_TW(true, 0);
+      } else {
+        if (zx) SU2_PLAQ_TW(false, 1);
+        else SU2_PLAQ_TW(false, 0);
+      }
+#undef SU2_PLAQ_TW
+      return (int)plaq_tw_blocks(f);
+    }
     if (flags & F_RESTORE)
       hipLaunchKernelGGL((k_plaq_su2_4d<double, true, QHalf>), grid, block, 0, st, hin,
                          (quat<double> *)Uout, f, w_t, w_s, mu_start, an, partials);
@@ -945,6 +969,28 @@ int launch_plaq(int group, int prec, const void *Uin, void *Uout, const Geo &g, 
     const dim3 grid(g.V / NTF), block(NTF);
     const bool rs = (flags & F_RESTORE) != 0;
     const int an = (flags & F_ANISO) ? 1 : 0;
+    if (plaq_tw_ok(f)) {
+      const bool zx = plaq_tw_zs(f);
+      const dim3 gt(plaq_tw_blocks(f));
+#define SU2_PLAQ_TW(T, R, Z)                                                               \
+  hipLaunchKernelGGL((k_plaq_su2_4d_tw<T, R, Z>), gt, block, 0, st, (const quat<T> *)Uin,  \
+                     (quat<T> *)Uout, f, w_t, w_s, mu_start, an, partials)
+#define SU2_PLAQ_TW2(T, R)                                                                 \
+  do {                                                                                     \
+    if (zx) SU2_PLAQ_TW(T, R, 1);                                                          \
+    else SU2_PLAQ_TW(T, R, 0);                                                             \
+  } while (0)
+      if (prec == 0) {
+        if (rs) SU2_PLAQ_TW2(double, true);
+        else SU2_PLAQ_TW2(double, false);
+      } else {
+        if (rs) SU2_PLAQ_TW2(float, true);
+        else SU2_PLAQ_TW2(float, false);
+      }
+#undef SU2_PLAQ_TW2
+#undef SU2_PLAQ_TW
+      return (int)plaq_tw_blocks(f);
+    }
 #define SU2_PLAQ(T, R)                                                                     \
   hipLaunchKernelGGL((k_plaq_su2_4d<T, R>), grid, block, 0, st, (const quat<T> *)Uin,     \
                      (quat<T> *)Uout, f, w_t, w_s, mu_start, an, partials)

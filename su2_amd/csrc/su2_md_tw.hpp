@@ -313,4 +313,123 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_twz(UI Uin, UO Uout, T *__res
   }
 }
 
+// Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) walking t:
+// the terms of k_plaq_su2_4d (su2_md_fast.hpp), a site's six planes in the
+// same order and arithmetic, with fewer link loads.  A thread keeps one
+// 3-volume site (o3 = q NTF + tid, the chunk of XCD b % 8) and walks the
+// chunk's Tc slices upwards: U_nu(s+t) (nu = x, y, z) of slice t are the next
+// slice's own links and stay in registers, and with ZS = 1 (64 % Lz == 0)
+// U_mu(s+z) (mu = t, x, y) comes from lane l + 1's own links — 10 link loads
+// per site instead of 16 (13 with ZS = 0).  Launch: 8 nb3 blocks of NTF
+// threads, one partial per block (the thread sums its slices' planes, then
+// the block sum).  RESTORE as k_plaq_su2_4d: every link reunitarised on load,
+// each site's own restored links written to Uout (record layout) once, by the
+// thread that owns the site.
+struct PlaqTwSite {
+  int o3, t0;
+  int upx, upy, upz;  // +x, +y, +z neighbour offsets of o3
+};
+// block b, thread tid of the plaquette t-walk: site o3 = (b / 8) NTF + tid of
+// the chunk of XCD b % 8
+__host__ __device__ __forceinline__ PlaqTwSite plaq_tw_site(const FastGeo &g, uint32_t b,
+                                                            uint32_t tid) {
+  PlaqTwSite f;
+  const uint32_t xcd = b & 7u, q = b >> 3;
+  const uint32_t o3 = q * (uint32_t)NTF + tid;
+  f.o3 = (int)o3;
+  f.t0 = (int)xcd * g.Tc;
+  const uint32_t x = fast_div(o3, g.m_plane);
+  const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
+  const uint32_t y = fast_div(rem3, g.m_Lz);
+  const uint32_t z = rem3 - y * (uint32_t)g.Lz;
+  f.upx = ((int)x == g.Lx - 1) ? -(g.Lx - 1) * g.plane : g.plane;
+  f.upy = ((int)y == g.Ly - 1) ? -(g.Ly - 1) * g.Lz : g.Lz;
+  f.upz = ((int)z == g.Lz - 1) ? -(g.Lz - 1) : 1;
+  return f;
+}
+// the lane holding the +z neighbour of lane `lane`'s site (64 % Lz == 0: a
+// wave holds 64 / Lz whole z rows)
+__host__ __device__ __forceinline__ int plaq_tw_zup(const FastGeo &g, int lane) {
+  const int zl = lane & (g.Lz - 1);
+  return (zl == g.Lz - 1) ? lane - (g.Lz - 1) : lane + 1;
+}
+
+template <typename T, bool RESTORE, int ZS, class UI = const quat<T> *>
+__global__ __launch_bounds__(NTF) void k_plaq_su2_4d_tw(UI Uin, quat<T> *__restrict__ Uout,
+                                                       FastGeo g, double w_t, double w_s,
+                                                       int mu_start, int aniso,
+                                                       double *__restrict__ partials) {
+  static_assert(ZS == 0 || ZS == 1, "plaquette t-walk: ZS 0 or 1");
+  __shared__ double lds[3 * NTF / 64];
+  const int V = g.V;
+  const PlaqTwSite f = plaq_tw_site(g, blockIdx.x, threadIdx.x);
+  const int upx = f.upx, upy = f.upy, upz = f.upz;
+  const int zup = (ZS == 1) ? plaq_tw_zup(g, (int)(threadIdx.x & 63)) : 0;
+  // link (m, i), reunitarised where RESTORE
+  auto lr = [&](int m, int i) {
+    quat<T> u = ldq(fld_add(Uin, (size_t)m * V), (uint32_t)i);
+    if (RESTORE) u = qrestore(u);
+    return u;
+  };
+  // one plane's term, as in k_plaq_su2_4d: 2 (U_mu(s) a) . (U_nu(s) b)
+  auto term = [&](const quat<T> &um, const quat<T> &a, const quat<T> &un, const quat<T> &b,
+                  int mu) {
+    const quat<double> X = qmul(to_d(um), to_d(a));
+    const quat<double> Y = qmul(to_d(un), to_d(b));
+    const double v = 2.0 * (X.ar * Y.ar + X.ai * Y.ai + X.br * Y.br + X.bi * Y.bi);
+    return aniso ? ((mu == 0) ? w_t : w_s) * v : v;
+  };
+  int s = f.t0 * g.vol3 + f.o3;
+  quat<T> own[4];
+#pragma unroll
+  for (int m = 1; m < 4; ++m) own[m] = lr(m, s);
+  double sum = 0.0;
+  for (int tt = 0; tt < g.Tc; ++tt) {
+    int up0, dn0;
+    tw_t_offsets(g, f.t0 + tt, up0, dn0);
+    own[0] = lr(0, s);
+    quat<T> nx[3];  // U_nu(s+t), nu = x, y, z: the next slice's own links
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nx[k] = lr(k + 1, s + up0);
+    if (RESTORE) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) sto(Uout + (size_t)m * V, (uint32_t)s, own[m]);
+    }
+    const quat<T> b01 = lr(0, s + upx);  // U_t(s+x)
+    const quat<T> b02 = lr(0, s + upy);  // U_t(s+y)
+    const quat<T> a12 = lr(2, s + upx);  // U_y(s+x)
+    const quat<T> b12 = lr(1, s + upy);  // U_x(s+y)
+    const quat<T> a13 = lr(3, s + upx);  // U_z(s+x)
+    const quat<T> a23 = lr(3, s + upy);  // U_z(s+y)
+    quat<T> b03, b13, b23;               // U_t(s+z), U_x(s+z), U_y(s+z)
+    if (ZS == 1) {
+      b03 = qshfl(own[0], zup);
+      b13 = qshfl(own[1], zup);
+      b23 = qshfl(own[2], zup);
+    } else {
+      b03 = lr(0, s + upz);
+      b13 = lr(1, s + upz);
+      b23 = lr(2, s + upz);
+    }
+    if (mu_start <= 0) {
+      sum += term(own[0], nx[0], own[1], b01, 0);
+      sum += term(own[0], nx[1], own[2], b02, 0);
+      sum += term(own[0], nx[2], own[3], b03, 0);
+    }
+    if (mu_start <= 1) {
+      sum += term(own[1], a12, own[2], b12, 1);
+      sum += term(own[1], a13, own[3], b13, 1);
+    }
+    if (mu_start <= 2) sum += term(own[2], a23, own[3], b23, 2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) own[k + 1] = nx[k];
+    s += up0;
+  }
+  const double r = block_sum<NTF>(sum, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// blocks (and partials) of a plaquette t-walk launch
+inline uint32_t plaq_tw_blocks(const FastGeo &g) { return 8u * (uint32_t)g.nb3; }
+
 }  // namespace su2k

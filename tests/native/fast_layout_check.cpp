@@ -106,6 +106,57 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
       }
     printf("%dx%dx%dx%d t-walk%s: ok (%u blocks)\n", L[0], L[1], L[2], L[3], wmu ? " wmu" : "", nbt);
   }
+  // the plaquette t-walk: 8 nb3 blocks, each thread one 3-volume site walking
+  // the Tc slices of its XCD's chunk; with 64 % Lz == 0 the +z neighbour's
+  // site is held by lane plaq_tw_zup of the same wave
+  if (g.tblock && !wmu && !xwalk) {
+    std::vector<unsigned char> seen_p((size_t)V, 0);
+    const uint32_t nbp = plaq_tw_blocks(g);
+    const bool zx = 64 % L[3] == 0;
+    for (uint32_t b = 0; b < nbp; ++b)
+      for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
+        const PlaqTwSite f = plaq_tw_site(g, b, tid);
+        if (f.o3 < 0 || f.o3 >= g.vol3 || f.t0 < 0 || f.t0 + g.Tc > L[0]) {
+          printf("FAIL plaquette t-walk range b=%u tid=%u o3=%d t0=%d\n", b, tid, f.o3, f.t0);
+          return 1;
+        }
+        if (zx) {
+          const uint32_t lane = tid & 63u, zl = (uint32_t)plaq_tw_zup(g, (int)lane);
+          if (zl >= 64u || plaq_tw_site(g, b, tid - lane + zl).o3 != f.o3 + f.upz) {
+            printf("FAIL plaquette t-walk +z lane of b=%u tid=%u\n", b, tid);
+            return 1;
+          }
+        }
+        int s = f.t0 * g.vol3 + f.o3;
+        for (int kt = 0; kt < g.Tc; ++kt) {
+          if (s < 0 || s >= V || seen_p[s]++) {
+            printf("FAIL plaquette t-walk site s=%d\n", s);
+            return 1;
+          }
+          int up0, dn0;
+          tw_t_offsets(g, f.t0 + kt, up0, dn0);
+          const int up[4] = {up0, f.upx, f.upy, f.upz};
+          int c[4], rest = s;
+          for (int d = 0; d < 4; ++d) {
+            c[d] = rest / str[d];
+            rest -= c[d] * str[d];
+          }
+          for (int d = 0; d < 4; ++d)
+            if (s + up[d] != s + ((c[d] + 1) % L[d] - c[d]) * str[d]) {
+              printf("FAIL plaquette t-walk neighbours of s=%d along %d\n", s, d);
+              return 1;
+            }
+          s += up0;
+        }
+      }
+    for (size_t i = 0; i < seen_p.size(); ++i)
+      if (seen_p[i] != 1) {
+        printf("FAIL plaquette t-walk site %zu not visited\n", i);
+        return 1;
+      }
+    printf("%dx%dx%dx%d plaquette t-walk%s: ok (%u blocks)\n", L[0], L[1], L[2], L[3],
+           zx ? " z-exchange" : "", nbp);
+  }
   // the thread-per-site shape: V / NTF blocks, every site exactly once
   const uint32_t nbs = (uint32_t)(V / NTF);
   if (g.tblock && nbs % 8 != 0) {
