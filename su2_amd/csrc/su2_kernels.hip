@@ -501,6 +501,26 @@ static int tw_pref() {
   if (v < 0) v = env_int("SU2GPU_TW", 1);
   return v;
 }
+// the t-walk's chunks (SU2GPU_TWC = C in 1, 2, 4, 8; default 2): C t-chunks of
+// Lt / C slices, XCD k walks chunk k % C of the 3-volume part k / C (8 / C
+// parts), so a thread's first slice, which loads what a slice below would
+// have passed on, comes once per Lt / C slices instead of once per Lt / 8
+// (32^4 FP64 141-142.5 against 143.4-145.9 us, FP32 53.6-54.2 against
+// 54.7-55.3 us, 64^4 FP64 2484 against 2508 us; alternating runs, one box).
+// Falls back to one chunk per XCD where the parts do not divide the blocks.
+static FastGeo tw_geo(const FastGeo &f, bool wmu) {
+  static int c = -1, cz = -1;
+  if (c < 0) c = env_int("SU2GPU_TWC", 2);
+  if (cz < 0) cz = env_int("SU2GPU_TWCZ", 8);
+  FastGeo g = f;
+  if (wmu) c = cz;
+  const int lc = c == 1 ? 0 : c == 2 ? 1 : c == 4 ? 2 : 3;
+  const int C = 1 << lc, P = 8 / C;
+  if (lc == 3 || f.Lt % C != 0 || (wmu ? f.nb64 : f.nb3) % P != 0 || f.Lt / C < 4) return g;
+  g.tw_lc = lc;
+  g.Tc = f.Lt / C;
+  return g;
+}
 // SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
 static int zs_pref() {
   static int v = -1;
@@ -522,32 +542,34 @@ static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch 
   // contraction, within the oracle parity either way)
   if constexpr (W && !A) {
     if (zs && f.tblock && f.Tc >= 4 && tw_pref()) {
-      const dim3 gt(tw_blocks(f)), bt(NTF);
+      const FastGeo g = tw_geo(f, false);
+      const dim3 gt(tw_blocks(g)), bt(NTF);
       if (nt == 2)
         hipLaunchKernelGGL((k_md_su2_4d_tw<double, 1, 2, UI, QHalfW>), gt, bt, 0, st, uin, uout,
-                           (double *)L.P, f, L.c);
+                           (double *)L.P, g, L.c);
       else if (nt == 1)
         hipLaunchKernelGGL((k_md_su2_4d_tw<double, 1, 1, UI, QHalfW>), gt, bt, 0, st, uin, uout,
-                           (double *)L.P, f, L.c);
+                           (double *)L.P, g, L.c);
       else
         hipLaunchKernelGGL((k_md_su2_4d_tw<double, 1, 0, UI, QHalfW>), gt, bt, 0, st, uin, uout,
-                           (double *)L.P, f, L.c);
+                           (double *)L.P, g, L.c);
       return;
     }
     // without the whole-row exchange (Lz does not divide 64): the t-walk with
     // the general exchange, one direction per wave (48^4: 776 against 832 us
     // in the same process; within the last bit of this step, as above)
     if (!zs && zs_pref() && zg_pref() != 0 && f.tblock && f.Tc >= 4 && tw_pref()) {
-      const dim3 gt(tw_blocks_w(f)), bt(NTF);
+      const FastGeo g = tw_geo(f, true);
+      const dim3 gt(tw_blocks_w(g)), bt(NTF);
       if (nt == 2)
         hipLaunchKernelGGL((k_md_su2_4d_twz<double, 2, UI, QHalfW>), gt, bt, 0, st, uin, uout,
-                           (double *)L.P, f, L.c);
+                           (double *)L.P, g, L.c);
       else if (nt == 1)
         hipLaunchKernelGGL((k_md_su2_4d_twz<double, 1, UI, QHalfW>), gt, bt, 0, st, uin, uout,
-                           (double *)L.P, f, L.c);
+                           (double *)L.P, g, L.c);
       else
         hipLaunchKernelGGL((k_md_su2_4d_twz<double, 0, UI, QHalfW>), gt, bt, 0, st, uin, uout,
-                           (double *)L.P, f, L.c);
+                           (double *)L.P, g, L.c);
       return;
     }
   }
@@ -649,12 +671,13 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
       // exchange, bit for bit (48^4: 386.6 against 395.6 us, 56^4: 723.5
       // against 745.2 us)
       if (!acc && L.Uout && fw.tblock && fw.Tc >= 4 && tw_pref()) {
+        const FastGeo g = tw_geo(fw, true);
         if (nt)
-          hipLaunchKernelGGL((k_md_su2_4d_twz<T, 2>), dim3(tw_blocks_w(fw)), dim3(NTF), 0, st,
-                             (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, fw, L.c);
+          hipLaunchKernelGGL((k_md_su2_4d_twz<T, 2>), dim3(tw_blocks_w(g)), dim3(NTF), 0, st,
+                             (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, g, L.c);
         else
-          hipLaunchKernelGGL((k_md_su2_4d_twz<T, 0>), dim3(tw_blocks_w(fw)), dim3(NTF), 0, st,
-                             (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, fw, L.c);
+          hipLaunchKernelGGL((k_md_su2_4d_twz<T, 0>), dim3(tw_blocks_w(g)), dim3(NTF), 0, st,
+                             (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, g, L.c);
         return true;
       }
       const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
@@ -684,8 +707,9 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     if constexpr (sizeof(T) == 4) {
       if (zs && !an && !nt && !acc && L.Uout && !L.in_half && !L.out_half && f.tblock &&
           f.Tc >= 4 && tw_pref()) {
-        hipLaunchKernelGGL((k_md_su2_4d_tw<T, 1, 0>), dim3(tw_blocks(f)), dim3(NTF), 0, st,
-                           (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, f, L.c);
+        const FastGeo g = tw_geo(f, false);
+        hipLaunchKernelGGL((k_md_su2_4d_tw<T, 1, 0>), dim3(tw_blocks(g)), dim3(NTF), 0, st,
+                           (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, g, L.c);
         return true;
       }
     }

@@ -76,6 +76,7 @@ struct FastGeo {
   int wmu;
   int nb64;       // vol3 / 64
   uint32_t m_nb64;
+  int tw_lc;      // t-walk: log2 of the t-chunks (3: one per XCD)
 };
 
 // floor(n / d) = umulhi(n, m) with m = floor(2^32 / d) + 1, exact whenever
@@ -198,6 +199,7 @@ inline bool make_fast_geo(const int L[4], int V, bool allow_tblock, FastGeo &f,
   // block counts divided: local < V / 64 / 8 by Tc, lb < V / 64 by nb64
   if (wmu && ((uint64_t)V / 64 * f.nb64 >= two32 || (uint64_t)V / 64 * f.Tc >= two32)) return false;
   f.m_nb64 = fast_div_magic(f.nb64);
+  f.tw_lc = 3;
   if (wmu) f.xwalk = 0;
   return true;
 }

@@ -39,9 +39,12 @@ __host__ __device__ __forceinline__ TwSite tw_site(const FastGeo &g, uint32_t b,
   TwSite f;
   const uint32_t xcd = b & 7u, local = b >> 3;
   f.mu = (int)(local & 3u);
-  const uint32_t o3 = (local >> 2) * (uint32_t)NTF + tid;
+  const uint32_t lc = (uint32_t)g.tw_lc;
+  const uint32_t chunk = xcd & ((1u << lc) - 1u), part = xcd >> lc;
+  const uint32_t nq = (uint32_t)g.nb3 >> (3u - lc);
+  const uint32_t o3 = (part * nq + (local >> 2)) * (uint32_t)NTF + tid;
   f.o3 = (int)o3;
-  f.t0 = (int)xcd * g.Tc;
+  f.t0 = (int)chunk * g.Tc;
   const uint32_t x = fast_div(o3, g.m_plane);
   const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
   const uint32_t y = fast_div(rem3, g.m_Lz);
@@ -56,14 +59,17 @@ __host__ __device__ __forceinline__ TwSite tw_site(const FastGeo &g, uint32_t b,
   return f;
 }
 
-// the mapping of k_md_su2_4d_twz: local = b / 8 = q, mu = wave, o3 = q 64 + lane
+// the mapping of k_md_su2_4d_twz: q = b / 8, mu = wave, o3 = (part nq + q) 64 + lane
 __host__ __device__ __forceinline__ TwSite tw_site_w(const FastGeo &g, uint32_t b, uint32_t tid) {
   TwSite f;
   const uint32_t xcd = b & 7u, q = b >> 3;
   f.mu = (int)(tid >> 6);
-  const uint32_t o3 = q * 64u + (tid & 63u);
+  const uint32_t lc = (uint32_t)g.tw_lc;
+  const uint32_t chunk = xcd & ((1u << lc) - 1u), part = xcd >> lc;
+  const uint32_t nq = (uint32_t)g.nb64 >> (3u - lc);
+  const uint32_t o3 = (part * nq + q) * 64u + (tid & 63u);
   f.o3 = (int)o3;
-  f.t0 = (int)xcd * g.Tc;
+  f.t0 = (int)chunk * g.Tc;
   const uint32_t x = fast_div(o3, g.m_plane);
   const uint32_t rem3 = o3 - x * (uint32_t)g.plane;
   const uint32_t y = fast_div(rem3, g.m_Lz);
@@ -77,7 +83,7 @@ __host__ __device__ __forceinline__ TwSite tw_site_w(const FastGeo &g, uint32_t 
   f.dn[3] = (z == 0) ? (g.Lz - 1) : -1;
   return f;
 }
-inline uint32_t tw_blocks_w(const FastGeo &g) { return 8u * (uint32_t)(g.vol3 / 64); }
+inline uint32_t tw_blocks_w(const FastGeo &g) { return (8u * (uint32_t)g.nb64) >> (3 - g.tw_lc); }
 
 // periodic t-neighbour offsets of slice t
 __host__ __device__ __forceinline__ void tw_t_offsets(const FastGeo &g, int t, int &up0,
@@ -87,7 +93,7 @@ __host__ __device__ __forceinline__ void tw_t_offsets(const FastGeo &g, int t, i
 }
 
 // blocks of a t-walk launch
-inline uint32_t tw_blocks(const FastGeo &g) { return 32u * (uint32_t)g.nb3; }
+inline uint32_t tw_blocks(const FastGeo &g) { return (32u * (uint32_t)g.nb3) >> (3 - g.tw_lc); }
 
 template <typename T, int MU, int ZS, int NT, class UI, class UO>
 __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ P,

@@ -62,17 +62,24 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
   // the t-walk shape (t-blocked layouts): 32 nb3 blocks, each thread one
   // (3-volume site, mu) walking the Tc slices of its XCD's chunk
   // (wmu: the mapping of the general-exchange t-walk, one direction per wave)
-  if (g.tblock) {
+  // (lc < 3: t-chunks of Lt / 2^lc slices, XCD k walking chunk k % 2^lc of
+  // 3-volume part k / 2^lc)
+  for (int lc = 3; lc >= 0 && g.tblock; --lc) {
+    const int C = 1 << lc, P = 8 / C;
+    if (lc < 3 && (L[0] % C != 0 || (wmu ? g.nb64 : g.nb3) % P != 0)) continue;
+    FastGeo gw = g;
+    gw.tw_lc = lc;
+    gw.Tc = L[0] / C;
     std::vector<unsigned char> seen_t((size_t)4 * V, 0);
-    const uint32_t nbt = wmu ? tw_blocks_w(g) : tw_blocks(g);
+    const uint32_t nbt = wmu ? tw_blocks_w(gw) : tw_blocks(gw);
     for (uint32_t b = 0; b < nbt; ++b)
       for (uint32_t tid = 0; tid < (uint32_t)NTF; ++tid) {
-        const TwSite f = wmu ? tw_site_w(g, b, tid) : tw_site(g, b, tid);
-        if (f.mu < 0 || f.mu > 3 || f.o3 < 0 || f.o3 >= g.vol3 || f.t0 < 0 || f.t0 + g.Tc > L[0]) {
+        const TwSite f = wmu ? tw_site_w(gw, b, tid) : tw_site(gw, b, tid);
+        if (f.mu < 0 || f.mu > 3 || f.o3 < 0 || f.o3 >= g.vol3 || f.t0 < 0 || f.t0 + gw.Tc > L[0]) {
           printf("FAIL t-walk range b=%u tid=%u mu=%d o3=%d t0=%d\n", b, tid, f.mu, f.o3, f.t0);
           return 1;
         }
-        for (int kt = 0; kt < g.Tc; ++kt) {
+        for (int kt = 0; kt < gw.Tc; ++kt) {
           const int t = f.t0 + kt, s = t * g.vol3 + f.o3;
           if (seen_t[(size_t)f.mu * V + s]++) {
             printf("FAIL t-walk link (s=%d, mu=%d) visited twice\n", s, f.mu);
@@ -104,7 +111,11 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
         printf("FAIL t-walk link %zu not visited\n", i);
         return 1;
       }
-    printf("%dx%dx%dx%d t-walk%s: ok (%u blocks)\n", L[0], L[1], L[2], L[3], wmu ? " wmu" : "", nbt);
+    if (lc == 3)
+      printf("%dx%dx%dx%d t-walk%s: ok (%u blocks)\n", L[0], L[1], L[2], L[3], wmu ? " wmu" : "", nbt);
+    else
+      printf("%dx%dx%dx%d t-walk%s %d chunks: ok (%u blocks)\n", L[0], L[1], L[2], L[3],
+             wmu ? " wmu" : "", C, nbt);
   }
   // the plaquette t-walk: 8 nb3 blocks, each thread one 3-volume site walking
   // the Tc slices of its XCD's chunk; with 64 % Lz == 0 the +z neighbour's

@@ -725,9 +725,10 @@ orc = Oracle()
 # (Lx, Ly, Lz, Lt) = (8, 16, 12, 8): Lz = 12 does not divide 64, the FP32
 # step's general z exchange (neighbours in another wave loaded by the lanes
 # that need them); (8, 8, 8, 32): 4 t-slices per XCD, the t-walk step;
-# (8, 8, 12, 32): both (the FP32 general-exchange t-walk)
+# (8, 8, 12, 32): both (the FP32 general-exchange t-walk); (16, 8, 8, 32): the
+# t-walk in two t-chunks of 16 slices over four 3-volume parts
 for dims in [(8, 8, 8, 8, 4), (2, 64, 64, 8, 4), (8, 16, 12, 8, 4), (8, 8, 8, 32, 4),
-             (8, 8, 12, 32, 4)]:
+             (8, 8, 12, 32, 4), (16, 8, 8, 32, 4)]:
     U = orc.hotstart(1, dims, 77, 0.3)
     P = orc.initnormal(1, dims, 78)
     Uw, _ = orc.integrate(1, dims, U, P, 2.4, 6, 5)
@@ -744,13 +745,14 @@ print("NT_CHILD_OK", flush=True)
 """
 
 
+_TW_GEOMS = (("z8", (8, 8, 8, 32, 4)), ("z12", (8, 8, 12, 32, 4)), ("z8p4", (16, 8, 8, 32, 4)))
 _TW_CHILD = r"""
 import sys, numpy as np
 sys.path.insert(0, sys.argv[1])
 import su2_amd as S
 from oracle.oracle import Oracle
 orc = Oracle()
-for tag, dims in (("z8", (8, 8, 8, 32, 4)), ("z12", (8, 8, 12, 32, 4))):
+for tag, dims in %r:
     U = orc.hotstart(1, dims, 91, 0.5)
     P = orc.initnormal(1, dims, 92)
     for name, prec in (("fp32", S.FP32), ("fp64", S.FP64)):
@@ -760,46 +762,54 @@ for tag, dims in (("z8", (8, 8, 8, 32, 4)), ("z12", (8, 8, 12, 32, 4))):
             lat.md_finish(True)
             np.save(sys.argv[2] + tag + name + ".npy", lat.get_links())
             print("TW_CHILD_OK", tag + name, repr(r["deltaH"]), flush=True)
-"""
+""" % (_TW_GEOMS,)
 
 
 def test_t_walk_step_parity(tmp_path, oracle):
     """The P-then-U step as a t-walk (k_md_su2_4d_tw: a thread walks the
-    t-slices of its XCD's chunk, keeping the staple links that neighbouring
-    slices share; on by default for lane-exchange lattices with >= 4 slices per
-    XCD, FP32 within the Infinity Cache or with the general z exchange) against
-    the one-launch-per-link step (SU2GPU_TW=0): one OMF2(6) trajectory on
-    (Lx, Ly, Lz, Lt) = (8, 8, 8, 32) and (8, 8, 12, 32) (SU2GPU_ZG=1: the FP32
-    general exchange, k_md_su2_4d_twz) in both forms, in two child processes
-    because the switches are read once per process.  FP32: links bit for bit
-    (the same FMA chains), dH within 1e-9 (the switch also selects the
-    plaquette sum's kernel, k_plaq_su2_4d_tw or one thread per site, which add
-    the same terms in another order); FP64: the last bit of some values may
-    differ (the compiler contracts the update's products per kernel), so both
-    against the oracle's trajectory, 1e-11 per link."""
+    t-slices of its t-chunk, keeping the staple links that neighbouring slices
+    share; on by default for lane-exchange lattices with >= 4 slices per XCD,
+    FP32 within the Infinity Cache or with the general z exchange) against the
+    one-launch-per-link step (SU2GPU_TW=0), and its default two t-chunks of
+    Lt / 2 slices (four 3-volume parts, where the 3-volume blocks divide by
+    four: the (Lx, Ly, Lz, Lt) = (16, 8, 8, 32) lattice) against one chunk per
+    XCD (SU2GPU_TWC=8): one OMF2(6) trajectory on (8, 8, 8, 32), (8, 8, 12, 32)
+    (SU2GPU_ZG=1: the FP32 general exchange, k_md_su2_4d_twz) and (16, 8, 8,
+    32) in both forms, in child processes because the switches are read once
+    per process.  FP32: links bit for bit (the same FMA chains), dH within
+    1e-9 (SU2GPU_TW also selects the plaquette sum's kernel, k_plaq_su2_4d_tw
+    or one thread per site, which add the same terms in another order); FP64:
+    the last bit of some values may differ (the compiler contracts the
+    update's products per code path), so each against the oracle's trajectory,
+    1e-11 per link."""
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    keys = [t + n for t in ("z8", "z12") for n in ("fp32", "fp64")]
+    keys = [t + n for t, _ in _TW_GEOMS for n in ("fp32", "fp64")]
+    cases = {"tw": {"SU2GPU_TW": "1"}, "tw8": {"SU2GPU_TW": "1", "SU2GPU_TWC": "8"},
+             "one": {"SU2GPU_TW": "0"}}
     outs = {}
-    for tw in ("1", "0"):
-        base = str(tmp_path / f"links_tw{tw}_")
-        e = dict(os.environ, SU2GPU_TW=tw, SU2GPU_ZG="1")
+    for case, env in cases.items():
+        base = str(tmp_path / f"links_{case}_")
+        e = {k: v for k, v in os.environ.items() if k not in ("SU2GPU_TW", "SU2GPU_TWC")}
+        e.update(SU2GPU_ZG="1", **env)
         out = subprocess.run([sys.executable, "-c", _TW_CHILD, root, base], env=e,
                              capture_output=True, text=True, timeout=300)
-        print(out.stdout)
+        print(case, out.stdout)
         assert out.returncode == 0, out.stdout + out.stderr
         dh = {ln.split()[1]: ln.split()[2] for ln in out.stdout.splitlines() if ln.startswith("TW_CHILD_OK")}
-        outs[tw] = {k: (np.load(base + k + ".npy"), dh[k]) for k in keys}
-    for tag, dims in (("z8", (8, 8, 8, 32, 4)), ("z12", (8, 8, 12, 32, 4))):
-        np.testing.assert_array_equal(outs["1"][tag + "fp32"][0], outs["0"][tag + "fp32"][0])
-        assert abs(float(outs["1"][tag + "fp32"][1]) - float(outs["0"][tag + "fp32"][1])) < 1e-9
+        outs[case] = {k: (np.load(base + k + ".npy"), dh[k]) for k in keys}
+    for tag, dims in _TW_GEOMS:
+        for case in ("tw8", "one"):
+            np.testing.assert_array_equal(outs["tw"][tag + "fp32"][0], outs[case][tag + "fp32"][0])
+        assert outs["tw"][tag + "fp32"][1] == outs["tw8"][tag + "fp32"][1]
+        assert abs(float(outs["tw"][tag + "fp32"][1]) - float(outs["one"][tag + "fp32"][1])) < 1e-9
         U = oracle.hotstart(1, dims, 91, 0.5)
         P = oracle.initnormal(1, dims, 92)
         Uw, _ = oracle.integrate(1, dims, U, P, 2.4, 6, 6)
-        for tw in ("1", "0"):
-            d = float(np.abs(outs[tw][tag + "fp64"][0] - Uw).max())
-            print(dims, "SU2GPU_TW", tw, "FP64 max|dU| vs oracle", d)
+        for case in cases:
+            d = float(np.abs(outs[case][tag + "fp64"][0] - Uw).max())
+            print(dims, case, "FP64 max|dU| vs oracle", d)
             assert d < 1e-11
 
 
