@@ -575,6 +575,36 @@ static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch 
       return;
     }
   }
+  // the first step (kinetic-energy and action partials) and the last,
+  // momentum-only step of a trajectory take the same walks
+  if constexpr (A) {
+    if (f.tblock && f.Tc >= 4 && tw_pref()) {
+      if (zs) {
+        const FastGeo g = tw_geo(f, false);
+        const dim3 gt(tw_blocks(g)), bt(NTF);
+#define SU2_TWA(N)                                                                              \
+  hipLaunchKernelGGL((k_md_su2_4d_tw_acc<double, W, 1, N, UI, QHalfW>), gt, bt, 0, st, uin, uout, \
+                     (double *)L.P, g, L.c, L.partials, L.pstride)
+        if (nt == 2) SU2_TWA(2);
+        else if (nt == 1) SU2_TWA(1);
+        else SU2_TWA(0);
+#undef SU2_TWA
+        return;
+      }
+      if (zs_pref() && zg_pref() != 0) {
+        const FastGeo g = tw_geo(f, true);
+        const dim3 gt(tw_blocks_w(g)), bt(NTF);
+#define SU2_TWA(N)                                                                              \
+  hipLaunchKernelGGL((k_md_su2_4d_twz_acc<double, W, N, UI, QHalfW>), gt, bt, 0, st, uin, uout,   \
+                     (double *)L.P, g, L.c, L.partials, L.pstride)
+        if (nt == 2) SU2_TWA(2);
+        else if (nt == 1) SU2_TWA(1);
+        else SU2_TWA(0);
+#undef SU2_TWA
+        return;
+      }
+    }
+  }
   const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
 #define SU2_HALF(Z, N)                                                                          \
   hipLaunchKernelGGL((k_md_su2_4d<double, W, A, 1, Z, false, N, UI, QHalfW>), grid, block, 0, st, \
@@ -672,14 +702,30 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
       // P-then-U steps with >= 4 t-slices per XCD: the t-walk with the same
       // exchange, bit for bit (48^4: 386.6 against 395.6 us, 56^4: 723.5
       // against 745.2 us)
-      if (!acc && L.Uout && fw.tblock && fw.Tc >= 4 && tw_pref()) {
+      if ((L.Uout || acc) && fw.tblock && fw.Tc >= 4 && tw_pref()) {
         const FastGeo g = tw_geo(fw, true);
-        if (nt)
-          hipLaunchKernelGGL((k_md_su2_4d_twz<T, 2>), dim3(tw_blocks_w(g)), dim3(NTF), 0, st,
-                             (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, g, L.c);
-        else
-          hipLaunchKernelGGL((k_md_su2_4d_twz<T, 0>), dim3(tw_blocks_w(g)), dim3(NTF), 0, st,
-                             (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, g, L.c);
+        const dim3 gt(tw_blocks_w(g)), bt(NTF);
+        const quat<T> *ui = (const quat<T> *)L.Uin;
+        quat<T> *uo = (quat<T> *)L.Uout;
+        if (!acc) {
+          if (nt)
+            hipLaunchKernelGGL((k_md_su2_4d_twz<T, 2>), gt, bt, 0, st, ui, uo, (T *)L.P, g, L.c);
+          else
+            hipLaunchKernelGGL((k_md_su2_4d_twz<T, 0>), gt, bt, 0, st, ui, uo, (T *)L.P, g, L.c);
+          return true;
+        }
+        // the first and the last step of a trajectory
+#define SU2_TWA(W, N)                                                                          \
+  hipLaunchKernelGGL((k_md_su2_4d_twz_acc<T, W, N, const quat<T> *, quat<T> *>), gt, bt, 0, st, \
+                     ui, uo, (T *)L.P, g, L.c, L.partials, L.pstride)
+        if (L.Uout) {
+          if (nt) SU2_TWA(true, 2);
+          else SU2_TWA(true, 0);
+        } else {
+          if (nt) SU2_TWA(false, 2);
+          else SU2_TWA(false, 0);
+        }
+#undef SU2_TWA
         return true;
       }
       const dim3 grid(4 * (L.geo.V / NTF)), block(NTF);
@@ -707,11 +753,20 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     // t-walk, the same arithmetic bit for bit with 10.75 instead of 14.5
     // link loads per link (32^4: 52.5 against 55.2 us)
     if constexpr (sizeof(T) == 4) {
-      if (zs && !an && !nt && !acc && L.Uout && !L.in_half && !L.out_half && f.tblock &&
+      if (zs && !an && !nt && (L.Uout || acc) && !L.in_half && !L.out_half && f.tblock &&
           f.Tc >= 4 && tw_pref()) {
         const FastGeo g = tw_geo(f, false);
-        hipLaunchKernelGGL((k_md_su2_4d_tw<T, 1, 0>), dim3(tw_blocks(g)), dim3(NTF), 0, st,
-                           (const quat<T> *)L.Uin, (quat<T> *)L.Uout, (T *)L.P, g, L.c);
+        const dim3 gt(tw_blocks(g)), bt(NTF);
+        const quat<T> *ui = (const quat<T> *)L.Uin;
+        quat<T> *uo = (quat<T> *)L.Uout;
+        if (!acc)
+          hipLaunchKernelGGL((k_md_su2_4d_tw<T, 1, 0>), gt, bt, 0, st, ui, uo, (T *)L.P, g, L.c);
+        else if (L.Uout)
+          hipLaunchKernelGGL((k_md_su2_4d_tw_acc<T, true, 1, 0, const quat<T> *, quat<T> *>), gt,
+                             bt, 0, st, ui, uo, (T *)L.P, g, L.c, L.partials, L.pstride);
+        else
+          hipLaunchKernelGGL((k_md_su2_4d_tw_acc<T, false, 1, 0, const quat<T> *, quat<T> *>), gt,
+                             bt, 0, st, ui, uo, (T *)L.P, g, L.c, L.partials, L.pstride);
         return true;
       }
     }

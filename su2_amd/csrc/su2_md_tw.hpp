@@ -95,10 +95,13 @@ __host__ __device__ __forceinline__ void tw_t_offsets(const FastGeo &g, int t, i
 // blocks of a t-walk launch
 inline uint32_t tw_blocks(const FastGeo &g) { return (32u * (uint32_t)g.nb3) >> (3 - g.tw_lc); }
 
-template <typename T, int MU, int ZS, int NT, class UI, class UO>
+// W: write the new link (false: the momentum update alone); ACC: add each
+// slice's squared old and new momenta and ReTr(U K) to acc (the step's
+// kinetic-energy and action partial sums)
+template <typename T, int MU, int ZS, int NT, class UI, class UO, bool W = true, bool ACC = false>
 __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ P,
                                                const FastGeo &g, const FusedCoef &c,
-                                               const TwSite &f) {
+                                               const TwSite &f, double (&acc)[3]) {
   static_assert(ZS >= 0 && ZS <= 2, "t-walk: ZS 0, 1 or 2");
   const int V = g.V;
   int up[4] = {0, f.up[1], f.up[2], f.up[3]}, dn[4] = {0, f.dn[1], f.dn[2], f.dn[3]};
@@ -275,8 +278,14 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
     const T pi0 = ldm<NT == 2>(P0, (uint32_t)s), pi1 = ldm<NT == 2>(P1, (uint32_t)s),
             pi2 = ldm<NT == 2>(P2, (uint32_t)s);
     T pn[3];
-    (void)md_su2_update<T, true, NT, UO>(u, K, pi0, pi1, pi2, P0, P1, P2,
-                                         fld_add(Uout, (size_t)MU * V), (uint32_t)s, c, pn);
+    const double mar = md_su2_update<T, W, NT, UO>(u, K, pi0, pi1, pi2, P0, P1, P2,
+                                                   fld_add(Uout, (size_t)MU * V), (uint32_t)s, c,
+                                                   pn);
+    if (ACC) {
+      acc[0] += sq3(pi0, pi1, pi2);
+      acc[1] += 2.0 * mar;
+      acc[2] += sq3(pn[0], pn[1], pn[2]);
+    }
     // what the next slice's link shares with this one
     if constexpr (MU != 0) {
       E0 = u;
@@ -298,11 +307,12 @@ template <typename T, int ZS, int NT, class UI = const quat<T> *, class UO = qua
 __global__ __launch_bounds__(NTF) void k_md_su2_4d_tw(UI Uin, UO Uout, T *__restrict__ P,
                                                     FastGeo g, FusedCoef c) {
   const TwSite f = tw_site(g, blockIdx.x, threadIdx.x);
+  double acc[3];
   switch (__builtin_amdgcn_readfirstlane(f.mu)) {
-    case 0: md_su2_tw_body<T, 0, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
-    case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
-    case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
-    default: md_su2_tw_body<T, 3, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    case 0: md_su2_tw_body<T, 0, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, ZS, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
   }
 }
 
@@ -311,12 +321,64 @@ template <typename T, int NT, class UI = const quat<T> *, class UO = quat<T> *>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d_twz(UI Uin, UO Uout, T *__restrict__ P,
                                                      FastGeo g, FusedCoef c) {
   const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
+  double acc[3];
   switch (__builtin_amdgcn_readfirstlane(f.mu)) {
-    case 0: md_su2_tw_body<T, 0, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
-    case 1: md_su2_tw_body<T, 1, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
-    case 2: md_su2_tw_body<T, 2, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
-    default: md_su2_tw_body<T, 3, 2, NT, UI, UO>(Uin, Uout, P, g, c, f); break;
+    case 0: md_su2_tw_body<T, 0, 2, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, 2, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, 2, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, 2, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
   }
+}
+
+// The block's partial sums of a t-walk with ACC: one block does the work of
+// Tc blocks of the one-launch step (Tc slices of NTF links), so its sums go
+// to entry Tc b of each row and entries Tc b + 1 .. Tc b + Tc - 1 are written
+// 0: the partial count, 4 V / NTF, is the one-launch step's.
+__device__ __forceinline__ void tw_partials(double (&acc)[3], double *lds, int Tc,
+                                            double *__restrict__ partials, int pstride) {
+  block_sum3<NTF>(acc[0], acc[1], acc[2], lds);
+  const size_t e0 = (size_t)Tc * blockIdx.x;
+  for (int i = (int)threadIdx.x; i < Tc; i += NTF) {
+    partials[0 * (size_t)pstride + e0 + i] = i == 0 ? acc[0] : 0.0;
+    partials[1 * (size_t)pstride + e0 + i] = i == 0 ? acc[1] : 0.0;
+    partials[2 * (size_t)pstride + e0 + i] = i == 0 ? acc[2] : 0.0;
+  }
+}
+
+// The first and the last step of a trajectory as t-walks: the step with the
+// kinetic-energy and action partial sums (ACCUM), and with W = false the
+// momentum update alone (the last step).
+template <typename T, bool W, int ZS, int NT, class UI, class UO>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_tw_acc(UI Uin, UO Uout, T *__restrict__ P,
+                                                        FastGeo g, FusedCoef c,
+                                                        double *__restrict__ partials,
+                                                        int pstride) {
+  __shared__ double lds[3 * NTF / 64];
+  const TwSite f = tw_site(g, blockIdx.x, threadIdx.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  switch (__builtin_amdgcn_readfirstlane(f.mu)) {
+    case 0: md_su2_tw_body<T, 0, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+  }
+  tw_partials(acc, lds, g.Tc, partials, pstride);
+}
+template <typename T, bool W, int NT, class UI, class UO>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_twz_acc(UI Uin, UO Uout, T *__restrict__ P,
+                                                         FastGeo g, FusedCoef c,
+                                                         double *__restrict__ partials,
+                                                         int pstride) {
+  __shared__ double lds[3 * NTF / 64];
+  const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  switch (__builtin_amdgcn_readfirstlane(f.mu)) {
+    case 0: md_su2_tw_body<T, 0, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+  }
+  tw_partials(acc, lds, g.Tc, partials, pstride);
 }
 
 // Plaquette sum (retr_sum_Wplaquettes, flat-gauge_energy.hpp:31-94) walking t:
