@@ -523,6 +523,14 @@ static FastGeo tw_geo(const FastGeo &f, bool wmu) {
   g.Tc = f.Lt / C;
   return g;
 }
+// SU2GPU_TWA = 0: the first and the last step of a trajectory (kinetic and
+// action partials, momentum-only last step) as one launch per link instead of
+// the t-walks
+static int twa_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_TWA", 1);
+  return v;
+}
 // SU2GPU_ZS = 0 disables the +-z lane exchange of the specialised step
 static int zs_pref() {
   static int v = -1;
@@ -578,7 +586,7 @@ static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch 
   // the first step (kinetic-energy and action partials) and the last,
   // momentum-only step of a trajectory take the same walks
   if constexpr (A) {
-    if (f.tblock && f.Tc >= 4 && tw_pref()) {
+    if (f.tblock && f.Tc >= 4 && tw_pref() && twa_pref()) {
       if (zs) {
         const FastGeo g = tw_geo(f, false);
         const dim3 gt(tw_blocks(g)), bt(NTF);
@@ -702,7 +710,7 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
       // P-then-U steps with >= 4 t-slices per XCD: the t-walk with the same
       // exchange, bit for bit (48^4: 386.6 against 395.6 us, 56^4: 723.5
       // against 745.2 us)
-      if ((L.Uout || acc) && fw.tblock && fw.Tc >= 4 && tw_pref()) {
+      if ((acc ? twa_pref() != 0 : L.Uout != nullptr) && fw.tblock && fw.Tc >= 4 && tw_pref()) {
         const FastGeo g = tw_geo(fw, true);
         const dim3 gt(tw_blocks_w(g)), bt(NTF);
         const quat<T> *ui = (const quat<T> *)L.Uin;
@@ -753,8 +761,8 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
     // t-walk, the same arithmetic bit for bit with 10.75 instead of 14.5
     // link loads per link (32^4: 52.5 against 55.2 us)
     if constexpr (sizeof(T) == 4) {
-      if (zs && !an && !nt && (L.Uout || acc) && !L.in_half && !L.out_half && f.tblock &&
-          f.Tc >= 4 && tw_pref()) {
+      if (zs && !an && !nt && (acc ? twa_pref() != 0 : L.Uout != nullptr) && !L.in_half &&
+          !L.out_half && f.tblock && f.Tc >= 4 && tw_pref()) {
         const FastGeo g = tw_geo(f, false);
         const dim3 gt(tw_blocks(g)), bt(NTF);
         const quat<T> *ui = (const quat<T> *)L.Uin;
