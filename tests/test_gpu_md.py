@@ -310,8 +310,10 @@ def test_fp32_form_u1_and_lower_dimensions(S, oracle, grp, dims, tol):
 
 # specialised SU(2) d=4 layouts (Lx, Ly, Lz, Lt): the plaquette sum walks t
 # with Tc = Lt / 8 slices per XCD = 1, 2, 4, with the +-z lane exchange
-# (Lz = 8, 16) and without it (Lz = 12)
-_PLAQ_GEOMS = [(8, 8, 8, 8, 4), (16, 16, 16, 16, 4), (8, 8, 8, 32, 4), (8, 8, 12, 32, 4)]
+# (Lz = 8, 16) and without it (Lz = 12); the MD steps walk t from Lt = 32 on,
+# in two t-chunks of 16 slices on (16, 8, 8, 32)
+_PLAQ_GEOMS = [(8, 8, 8, 8, 4), (16, 16, 16, 16, 4), (8, 8, 8, 32, 4), (8, 8, 12, 32, 4),
+               (16, 8, 8, 32, 4)]
 
 
 @pytest.mark.parametrize("dims", _PLAQ_GEOMS)
@@ -335,23 +337,34 @@ def test_specialised_plaquette_sums(S, oracle, dims, prec):
 
 
 @pytest.mark.parametrize("dims", _PLAQ_GEOMS)
-def test_fp64_trajectory_restore_action(S, oracle, dims):
-    """The FP64 trajectory's last kernel reunitarises the final links
-    (integrator.hh:48,111) and sums their action (md_update.hh:47-51) in one
-    launch on the specialised layouts: S_new equals the oracle's action of the
-    links it leaves, S_old the action of the start links, and the final links
-    are unit quaternions."""
+@pytest.mark.parametrize("prec", [0, 1])
+def test_trajectory_sums_on_specialised_layouts(S, oracle, dims, prec):
+    """The sums of a trajectory on the specialised layouts, where the first
+    step (kinetic-energy partials of the start momenta) and the last,
+    momentum-only step (those of the final momenta) walk t with Lt >= 32: K_old
+    and K_new equal the oracle's kinetic energies of the start and final
+    momenta (monomial.hh:33-47).  FP64: the last kernel reunitarises the final
+    links (integrator.hh:48,111) and sums their action (md_update.hh:47-51) in
+    one launch; S_new equals the oracle's action of the links it leaves, S_old
+    the action of the start links, and the final links are unit quaternions."""
     U = _start(oracle, 1, dims)
     P = oracle.initnormal(1, dims, SEED + 5)
-    with S.Lattice(S.SU2, dims, 2.4) as lat:
+    with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
         lat.set_links(U)
+        Us = lat.get_links()
+        lat.set_momenta(P)
+        Ps = lat.get_momenta()
         r = lat.md_trajectory(P, S.OMF2, 6)
         lat.md_finish(True)
         Uend = lat.get_links()
-    assert r["gauge_old"] == pytest.approx(oracle.action(1, dims, U, 2.4, compensated=True), rel=1e-13)
+        Pend = lat.get_momenta()
+    assert r["kin_old"] == pytest.approx(oracle.kinetic(1, dims, Ps, compensated=True), rel=1e-13)
+    assert r["kin_new"] == pytest.approx(oracle.kinetic(1, dims, Pend, compensated=True), rel=1e-13)
+    assert r["gauge_old"] == pytest.approx(oracle.action(1, dims, Us, 2.4, compensated=True), rel=1e-13)
     assert r["gauge_new"] == pytest.approx(oracle.action(1, dims, Uend, 2.4, compensated=True), rel=1e-13)
-    q = Uend.reshape(-1, 4)
-    assert np.abs((q ** 2).sum(1) - 1.0).max() < 1e-14
+    if prec == 0:
+        q = Uend.reshape(-1, 4)
+        assert np.abs((q ** 2).sum(1) - 1.0).max() < 1e-14
 
 
 @pytest.mark.parametrize("dims", [(8, 8, 8, 8, 4), (16, 8, 8, 8, 4), (4, 4, 4, 4, 4)])
