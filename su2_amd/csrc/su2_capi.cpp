@@ -84,6 +84,11 @@ struct su2gpu_lattice {
   // split md_update state
   bool pending = false;
   int pend_old = -1, pend_new = -1;
+  // device momenta of md_update not drawn yet: the trajectory's first fused
+  // step draws them (FusedLaunch::gen), or flush_gen before anything else
+  // reads P
+  bool gen_pending = false;
+  uint64_t gen_seed = 0, gen_traj = 0;
   // FP32 storage emulation of the FP64 form (su2gpu_set_fp32_storage_emulation):
   // bit 0 links, bit 1 momenta
   int r32 = 0;
@@ -252,8 +257,13 @@ int fused_ptr(L *h, const void *Uin, void *Uout, void *P, su2k::FusedCoef c, int
 // fused "P then U" (Uout >= 0) or momentum-only (Uout < 0) step.  allow_half
 // (the integrators only): where the step supports it, the new links go out in
 // the half-field layout; links already in it are read in it.
+int flush_gen(L *h);
 int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row,
           bool allow_half = false) {
+  // pending device momenta: drawn inside a first P-then-U step with partial
+  // sums, else by their own launch now
+  const bool gen = h->gen_pending && P == h->P && uout >= 0 && accum_row >= 0;
+  if (h->gen_pending && !gen) CK(flush_gen(h));
   su2k::FusedLaunch fl;
   fl.Uin = h->U[uin];
   fl.Uout = uout >= 0 ? h->U[uout] : nullptr;
@@ -271,6 +281,12 @@ int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row,
     fl.out_half = uout >= 0 ? 1 : 0;
   else if (fl.in_half)
     return fail(SU2GPU_ESTATE, "links in the half-field layout where the step cannot read them");
+  if (gen) {
+    fl.gen = 1;
+    fl.c.gseed = h->gen_seed;
+    fl.c.gtraj = h->gen_traj;
+    h->gen_pending = false;
+  }
   bool ok;
   {
     Prof pr(h, uout >= 0 ? "md_fused" : "md_p_only");
@@ -279,6 +295,18 @@ int fused(L *h, int uin, int uout, void *P, su2k::FusedCoef c, int accum_row,
   if (!ok) return fail(SU2GPU_ESTATE, "fused step: link layout not supported for this lattice");
   if (uout >= 0) h->half[uout] = fl.out_half;
   return last_launch("md_fused");
+}
+
+// pending device momenta drawn by their own launch
+int flush_gen(L *h) {
+  if (!h->gen_pending) return SU2GPU_OK;
+  h->gen_pending = false;
+  {
+    Prof pr(h, "momenta");
+    su2k::launch_momenta_philox(h->prec, h->P, h->V, h->nd, h->nm, h->gen_seed, h->gen_traj,
+                                h->stream);
+  }
+  return last_launch("momenta");
 }
 
 // links of buffer b in the record layout: a half-field buffer is converted
@@ -496,6 +524,7 @@ int run_integrator(L *h, int kind, size_t nsteps, double tau, size_t exponent,
     } else {
       // a link update without a momentum step before it (no integrator of
       // integrator.hh has one; kept general): the generic kernel, record layout
+      CK(flush_gen(h));
       {
         std::initializer_list<int> keep = protect;
         CK(to_record(h, &cur, keep));
@@ -1069,6 +1098,7 @@ static int trajectory_sums(L *h, int kind, size_t nsteps, double tau, size_t exp
   const double w_t = h->aniso ? 1.0 / h->xi : 1.0, w_s = h->aniso ? h->xi : 1.0;
   const int pflags = h->aniso ? su2k::F_ANISO : 0;
   h->cur = u_start;
+  if (h->r32 & 2) CK(flush_gen(h));
   // FP32 storage emulation: the fields an FP32 store would hold at the start
   if (h->r32 & 1) su2k::launch_round_f32((double *)h->U[u_start], (size_t)h->nlinks * h->ncomp, h->stream);
   if (h->r32 & 2) su2k::launch_round_f32((double *)h->P, (size_t)h->nlinks * h->nm, h->stream);
@@ -1168,15 +1198,22 @@ int su2gpu_md_update(su2gpu_lattice *h, uint64_t seed, int integrator, size_t ns
     host_normals(h, engine);
     CK(set_mom_host(h, h->P, h->hstage));
   } else if (momenta_source == SU2GPU_MOMENTA_DEVICE_PHILOX) {
-    {
-      Prof pr(h, "momenta");
-      su2k::launch_momenta_philox(h->prec, h->P, h->V, h->nd, h->nm, seed, 0, h->stream);
-    }
-    CK(last_launch("momenta"));
+    // drawn by the trajectory's first step where that walks t (the Philox
+    // stream of launch_momenta_philox either way), else by their own launch
+    h->gen_pending = true;
+    h->gen_seed = seed;
+    h->gen_traj = 0;
   } else {
     return fail(SU2GPU_EINVAL, "bad momenta source");
   }
-  CK(su2gpu_md_trajectory(h, nullptr, integrator, nsteps, tau, exponent, revtest, res));
+  const int rt = su2gpu_md_trajectory(h, nullptr, integrator, nsteps, tau, exponent, revtest, res);
+  if (h->gen_pending) {
+    // not consumed (an error before the first step): P holds what it would
+    h->gen_pending = false;
+    CK(rt);
+    return fail(SU2GPU_ESTATE, "device momenta not drawn");
+  }
+  CK(rt);
   int accept = 1;
   if (res->deltaH > 0) {
     double u;

@@ -939,7 +939,115 @@ template <class G> static bool launch_fused_g(const FusedLaunch &L, hipStream_t 
   }
 }
 
+// SU2GPU_GENFUSE = 0: device momenta always drawn by their own launch
+static int genfuse_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_GENFUSE", 1);
+  return v;
+}
+
+// The first P-then-U step of a trajectory that draws its momenta itself
+// (FusedLaunch::gen): the walks of launch_half / launch_fast with GEN, under
+// the conditions under which those take the first step's walk.  Returns false
+// (nothing launched) elsewhere; the caller then draws the momenta first.
+static bool launch_first_step_gen(int group, int prec, const FusedLaunch &L, hipStream_t st) {
+  if (!genfuse_pref() || group != 1 || L.geo.nd != 4 || !L.Uout || L.Pout) return false;
+  if (!(L.c.flags & F_ACCUM)) return false;
+  if (L.c.flags & (F_PZERO | F_ROUND | F_R32L | F_R32P | F_ANISO)) return false;
+  if (fast_level() < (prec == 0 ? 2 : 1) || fast_shape() == 1 || fused_mode(L.geo) == 0) return false;
+  if (!tw_pref() || !twa_pref()) return false;
+  FastGeo f;
+  if (!fast_geo_for(L.geo, f) || !f.tblock || f.Tc < 4) return false;
+  const bool zs = zs_pref() && (64 % f.Lz == 0);
+  const int ntp = nt_pref();
+  const double fields =
+      4.0 * L.geo.V * (2.0 * (prec == 0 ? sizeof(quat<double>) : sizeof(quat<float>)) +
+                       3.0 * (prec == 0 ? 8.0 : 4.0));
+  const bool nt = ntp >= 0 ? ntp != 0 : fields > 256.0 * 1024 * 1024;
+  const dim3 bt(NTF);
+  if (prec == 0) {
+    // FP64: the walks write the half-field layout
+    if (!L.out_half) return false;
+    const int ntl = nt ? ((ntp < 0 && fields > 1024.0 * 1024 * 1024) || ntp == 2 ? 2 : 1) : 0;
+    const size_t nl = 4 * (size_t)L.geo.V;
+    su2_d2 *oa = (su2_d2 *)L.Uout;
+    const QHalfW hout{oa, oa + nl};
+    const su2_d2 *ia = (const su2_d2 *)L.Uin;
+    const QHalf hin{ia, ia + nl};
+    const quat<double> *rin = (const quat<double> *)L.Uin;
+    double *P = (double *)L.P;
+    if (zs) {
+      const FastGeo g = tw_geo(f, false);
+      const dim3 gt(tw_blocks(g));
+#define SU2_GEN(UI, u, N)                                                                      \
+  hipLaunchKernelGGL((k_md_su2_4d_tw_acc<double, true, 1, N, UI, QHalfW, true>), gt, bt, 0, st, \
+                     u, hout, P, g, L.c, L.partials, L.pstride)
+      if (L.in_half) {
+        if (ntl == 2) SU2_GEN(QHalf, hin, 2);
+        else if (ntl == 1) SU2_GEN(QHalf, hin, 1);
+        else SU2_GEN(QHalf, hin, 0);
+      } else {
+        if (ntl == 2) SU2_GEN(const quat<double> *, rin, 2);
+        else if (ntl == 1) SU2_GEN(const quat<double> *, rin, 1);
+        else SU2_GEN(const quat<double> *, rin, 0);
+      }
+#undef SU2_GEN
+      return true;
+    }
+    if (!zs_pref() || zg_pref() == 0) return false;
+    const FastGeo g = tw_geo(f, true);
+    const dim3 gt(tw_blocks_w(g));
+#define SU2_GEN(UI, u, N)                                                                       \
+  hipLaunchKernelGGL((k_md_su2_4d_twz_acc<double, true, N, UI, QHalfW, true>), gt, bt, 0, st, u, \
+                     hout, P, g, L.c, L.partials, L.pstride)
+    if (L.in_half) {
+      if (ntl == 2) SU2_GEN(QHalf, hin, 2);
+      else if (ntl == 1) SU2_GEN(QHalf, hin, 1);
+      else SU2_GEN(QHalf, hin, 0);
+    } else {
+      if (ntl == 2) SU2_GEN(const quat<double> *, rin, 2);
+      else if (ntl == 1) SU2_GEN(const quat<double> *, rin, 1);
+      else SU2_GEN(const quat<double> *, rin, 0);
+    }
+#undef SU2_GEN
+    return true;
+  }
+  // FP32: record-layout links
+  if (L.in_half || L.out_half) return false;
+  const quat<float> *ui = (const quat<float> *)L.Uin;
+  quat<float> *uo = (quat<float> *)L.Uout;
+  float *P = (float *)L.P;
+  if (zs) {
+    if (nt) return false;
+    const FastGeo g = tw_geo(f, false);
+    hipLaunchKernelGGL((k_md_su2_4d_tw_acc<float, true, 1, 0, const quat<float> *, quat<float> *, true>),
+                       dim3(tw_blocks(g)), bt, 0, st, ui, uo, P, g, L.c, L.partials, L.pstride);
+    return true;
+  }
+  const int zg = zg_pref();
+  FastGeo fw;
+  if (!zs_pref() || !(zg >= 0 ? zg != 0 : nt) || !fast_geo_for(L.geo, fw, true) || !fw.tblock ||
+      fw.Tc < 4)
+    return false;
+  const FastGeo g = tw_geo(fw, true);
+  if (nt)
+    hipLaunchKernelGGL((k_md_su2_4d_twz_acc<float, true, 2, const quat<float> *, quat<float> *, true>),
+                       dim3(tw_blocks_w(g)), bt, 0, st, ui, uo, P, g, L.c, L.partials, L.pstride);
+  else
+    hipLaunchKernelGGL((k_md_su2_4d_twz_acc<float, true, 0, const quat<float> *, quat<float> *, true>),
+                       dim3(tw_blocks_w(g)), bt, 0, st, ui, uo, P, g, L.c, L.partials, L.pstride);
+  return true;
+}
+
 bool launch_fused(int group, int prec, const FusedLaunch &L, hipStream_t st) {
+  if (L.gen) {
+    if (launch_first_step_gen(group, prec, L, st)) return true;
+    launch_momenta_philox(prec, L.P, L.geo.V, L.geo.nd, group == 1 ? 3 : 1, L.c.gseed, L.c.gtraj,
+                          st);
+    FusedLaunch L2 = L;
+    L2.gen = 0;
+    return launch_fused(group, prec, L2, st);
+  }
   if (group == 1) {
     if (prec == 0) return launch_fused_g<GSU2<double>>(L, st);
     return launch_fused_g<GSU2<float>>(L, st);

@@ -34,6 +34,9 @@ struct FusedCoef {
   double w_s;      // weight of purely spatial ones (xi)
   double round_n;  // rounding multiplier 10^exponent (F_ROUND)
   int flags;
+  // the first step drawing its momenta (FusedLaunch::gen): the Philox stream
+  // of launch_momenta_philox with this seed and trajectory number
+  uint64_t gseed = 0, gtraj = 0;
 };
 
 struct FusedLaunch {
@@ -48,6 +51,10 @@ struct FusedLaunch {
   // FP64 SU(2) half-field layout (su2_md_fast.hpp QHalf) of the links read /
   // written; only where fused_half_ok says the step supports it
   int in_half = 0, out_half = 0;
+  // 1: P holds no momenta yet; they are the Philox momenta of c.gseed /
+  // c.gtraj (launch_momenta_philox), drawn inside the step where it walks t,
+  // else by launch_momenta_philox before it
+  int gen = 0;
 };
 
 // false if a half-field layout was asked for where the step does not support it

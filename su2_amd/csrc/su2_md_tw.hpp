@@ -97,8 +97,11 @@ inline uint32_t tw_blocks(const FastGeo &g) { return (32u * (uint32_t)g.nb3) >> 
 
 // W: write the new link (false: the momentum update alone); ACC: add each
 // slice's squared old and new momenta and ReTr(U K) to acc (the step's
-// kinetic-energy and action partial sums)
-template <typename T, int MU, int ZS, int NT, class UI, class UO, bool W = true, bool ACC = false>
+// kinetic-energy and action partial sums); GEN: the old momenta are not
+// loaded but drawn, link (s, mu) from the Philox counter of
+// k_momenta_philox (s 4 + mu, 0, traj), so they equal that kernel's bit for bit
+template <typename T, int MU, int ZS, int NT, class UI, class UO, bool W = true, bool ACC = false,
+          bool GEN = false>
 __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ P,
                                                const FastGeo &g, const FusedCoef &c,
                                                const TwSite &f, double (&acc)[3]) {
@@ -275,8 +278,19 @@ __device__ __forceinline__ void md_su2_tw_body(UI Uin, UO Uout, T *__restrict__ 
     T *__restrict__ P0 = P + (size_t)(MU * 3 + 0) * V;
     T *__restrict__ P1 = P + (size_t)(MU * 3 + 1) * V;
     T *__restrict__ P2 = P + (size_t)(MU * 3 + 2) * V;
-    const T pi0 = ldm<NT == 2>(P0, (uint32_t)s), pi1 = ldm<NT == 2>(P1, (uint32_t)s),
-            pi2 = ldm<NT == 2>(P2, (uint32_t)s);
+    T pi0, pi1, pi2;
+    if constexpr (GEN) {
+      T z[4];
+      philox_normals4<T>((uint32_t)(s * 4 + MU), (uint32_t)(c.gtraj & 0xffffffffu),
+                         (uint32_t)(c.gtraj >> 32), c.gseed, z);
+      pi0 = z[0];
+      pi1 = z[1];
+      pi2 = z[2];
+    } else {
+      pi0 = ldm<NT == 2>(P0, (uint32_t)s);
+      pi1 = ldm<NT == 2>(P1, (uint32_t)s);
+      pi2 = ldm<NT == 2>(P2, (uint32_t)s);
+    }
     T pn[3];
     const double mar = md_su2_update<T, W, NT, UO>(u, K, pi0, pi1, pi2, P0, P1, P2,
                                                    fld_add(Uout, (size_t)MU * V), (uint32_t)s, c,
@@ -347,8 +361,9 @@ __device__ __forceinline__ void tw_partials(double (&acc)[3], double *lds, int T
 
 // The first and the last step of a trajectory as t-walks: the step with the
 // kinetic-energy and action partial sums (ACCUM), and with W = false the
-// momentum update alone (the last step).
-template <typename T, bool W, int ZS, int NT, class UI, class UO>
+// momentum update alone (the last step); GEN: the first step draws the
+// trajectory's momenta itself.
+template <typename T, bool W, int ZS, int NT, class UI, class UO, bool GEN = false>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d_tw_acc(UI Uin, UO Uout, T *__restrict__ P,
                                                         FastGeo g, FusedCoef c,
                                                         double *__restrict__ partials,
@@ -357,14 +372,14 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_tw_acc(UI Uin, UO Uout, T *__
   const TwSite f = tw_site(g, blockIdx.x, threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
   switch (__builtin_amdgcn_readfirstlane(f.mu)) {
-    case 0: md_su2_tw_body<T, 0, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
-    case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
-    case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
-    default: md_su2_tw_body<T, 3, ZS, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    case 0: md_su2_tw_body<T, 0, ZS, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, ZS, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, ZS, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, ZS, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
   }
   tw_partials(acc, lds, g.Tc, partials, pstride);
 }
-template <typename T, bool W, int NT, class UI, class UO>
+template <typename T, bool W, int NT, class UI, class UO, bool GEN = false>
 __global__ __launch_bounds__(NTF) void k_md_su2_4d_twz_acc(UI Uin, UO Uout, T *__restrict__ P,
                                                          FastGeo g, FusedCoef c,
                                                          double *__restrict__ partials,
@@ -373,10 +388,10 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_twz_acc(UI Uin, UO Uout, T *_
   const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
   switch (__builtin_amdgcn_readfirstlane(f.mu)) {
-    case 0: md_su2_tw_body<T, 0, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
-    case 1: md_su2_tw_body<T, 1, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
-    case 2: md_su2_tw_body<T, 2, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
-    default: md_su2_tw_body<T, 3, 2, NT, UI, UO, W, true>(Uin, Uout, P, g, c, f, acc); break;
+    case 0: md_su2_tw_body<T, 0, 2, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, 2, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, 2, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, 2, NT, UI, UO, W, true, GEN>(Uin, Uout, P, g, c, f, acc); break;
   }
   tw_partials(acc, lds, g.Tc, partials, pstride);
 }

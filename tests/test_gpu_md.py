@@ -826,6 +826,70 @@ def test_t_walk_step_parity(tmp_path, oracle):
             assert d < 1e-11
 
 
+_GEN_CHILD = r"""
+import sys, numpy as np
+sys.path.insert(0, sys.argv[1])
+import su2_amd as S
+from oracle.oracle import Oracle
+orc = Oracle()
+for tag, dims in %r:
+    U = orc.hotstart(1, dims, 93, 0.5)
+    for name, prec in (("fp32", S.FP32), ("fp64", S.FP64)):
+        with S.Lattice(S.SU2, dims, 2.4, precision=prec) as lat:
+            lat.set_links(U)
+            r1 = lat.md_update(501, S.OMF2, 6, momenta=S.MOMENTA_DEVICE)
+            r2 = lat.md_update(502, S.OMF2, 6, momenta=S.MOMENTA_DEVICE, revtest=True)
+            np.save(sys.argv[2] + tag + name + "_U.npy", lat.get_links())
+            np.save(sys.argv[2] + tag + name + "_P.npy", lat.get_momenta())
+            vals = [r1[k] for k in ("deltaH", "kin_old", "kin_new")] + \
+                   [r2[k] for k in ("deltaH", "kin_old", "kin_new", "deltadeltaH")]
+            print("GEN_CHILD_OK", tag + name, " ".join(repr(v) for v in vals), flush=True)
+""" % (_TW_GEOMS,)
+
+
+def test_momenta_drawn_in_first_step_parity(tmp_path):
+    """Device momenta of md_update (the Philox stream of k_momenta_philox)
+    drawn inside the trajectory's first step where that walks t
+    (k_md_su2_4d_tw_acc / _twz_acc with GEN: link (s, mu) draws counter
+    (4 s + mu, 0, traj), the momenta kernel's) against their own launch before
+    the first step (SU2GPU_GENFUSE=0): two OMF2(6) trajectories, the second
+    with the reversibility test, on (Lx, Ly, Lz, Lt) = (8, 8, 8, 32),
+    (8, 8, 12, 32) (SU2GPU_ZG=1: the general-exchange walks of both forms) and
+    (16, 8, 8, 32), in child processes because the switch is read once per
+    process.  FP32: links, final momenta, dH, K_old, K_new and ddH bit for bit
+    (the same arithmetic on the same momenta); FP64: within the last bits (the
+    compiler contracts the update's products per kernel), links and momenta
+    1e-12, sums 1e-9."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = {}
+    for case in ("1", "0"):
+        base = str(tmp_path / f"gen{case}_")
+        e = {k: v for k, v in os.environ.items() if not k.startswith("SU2GPU_")}
+        e.update(SU2GPU_GENFUSE=case, SU2GPU_ZG="1")
+        out = subprocess.run([sys.executable, "-c", _GEN_CHILD, root, base], env=e,
+                             capture_output=True, text=True, timeout=300)
+        print(case, out.stdout)
+        assert out.returncode == 0, out.stdout + out.stderr
+        vals = {ln.split()[1]: [float(x) for x in ln.split()[2:]] for ln in out.stdout.splitlines()
+                if ln.startswith("GEN_CHILD_OK")}
+        outs[case] = (base, vals)
+    for tag, _ in _TW_GEOMS:
+        for name in ("fp32", "fp64"):
+            k = tag + name
+            a = [np.load(outs[c][0] + k + f) for c in ("1", "0") for f in ("_U.npy", "_P.npy")]
+            va, vb = outs["1"][1][k], outs["0"][1][k]
+            if name == "fp32":
+                np.testing.assert_array_equal(a[0], a[2])
+                np.testing.assert_array_equal(a[1], a[3])
+                assert va == vb
+            else:
+                assert float(np.abs(a[0] - a[2]).max()) < 1e-12
+                assert float(np.abs(a[1] - a[3]).max()) < 1e-12
+                np.testing.assert_allclose(va, vb, rtol=0, atol=1e-9)
+
+
 @pytest.mark.parametrize("env", [{"SU2GPU_NT": "1", "SU2GPU_XWALK": "1"},
                                  {"SU2GPU_NT": "1", "SU2GPU_XWALK": "0"},
                                  {"SU2GPU_NT": "2"},
