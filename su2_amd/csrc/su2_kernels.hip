@@ -783,7 +783,12 @@ static bool launch_fast(const FusedLaunch &L, hipStream_t st) {
       if constexpr (sizeof(T) == 8) {
         // momenta loaded non-temporally too where the fields exceed 1 GiB
         // (48^4: 832 against 850 us; 32^4, 368 MB: 161 against 148 us)
-        const int ntl = nt ? ((ntp < 0 && fields > 1024.0 * 1024 * 1024) || ntp == 2 ? 2 : 1) : 0;
+        int ntl = nt ? ((ntp < 0 && fields > 1024.0 * 1024 * 1024) || ntp == 2 ? 2 : 1) : 0;
+        // the last, momentum-only step loads the momenta non-temporally too:
+        // more of the links that the final restore + action launch reads next
+        // stay in the Infinity Cache (32^4: that launch 71-72 against 81-84 us,
+        // the trajectory 5.851-5.860 against 5.884-5.886 ms on one box)
+        if (!L.Uout && ntl == 1) ntl = 2;
         const size_t nl = 4 * (size_t)L.geo.V;
         const su2_d2 *ia = (const su2_d2 *)L.Uin;
         su2_d2 *oa = (su2_d2 *)L.Uout;
