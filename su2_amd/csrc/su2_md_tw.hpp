@@ -330,10 +330,12 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_tw(UI Uin, UO Uout, T *__rest
   }
 }
 
-// the t-walk with the general +-z exchange, one direction per wave
+// the t-walk with the general +-z exchange, one direction per wave.  FP32:
+// at least 4 waves per SIMD (128 instead of 132 VGPRs, no spills; 48^4 383.7
+// against 395.4 us, 40^4 187.8 against 194.6 us, 56^4 725 against 743 us)
 template <typename T, int NT, class UI = const quat<T> *, class UO = quat<T> *>
-__global__ __launch_bounds__(NTF) void k_md_su2_4d_twz(UI Uin, UO Uout, T *__restrict__ P,
-                                                     FastGeo g, FusedCoef c) {
+__global__ __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_md_su2_4d_twz(UI Uin, UO Uout, T *__restrict__ P, FastGeo g, FusedCoef c) {
   const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
   double acc[3];
   switch (__builtin_amdgcn_readfirstlane(f.mu)) {
@@ -363,11 +365,11 @@ __device__ __forceinline__ void tw_partials(double (&acc)[3], double *lds, int T
 // kinetic-energy and action partial sums (ACCUM), and with W = false the
 // momentum update alone (the last step); GEN: the first step draws the
 // trajectory's momenta itself.
+// FP32: at least 4 waves per SIMD, as k_md_su2_4d_twz
 template <typename T, bool W, int ZS, int NT, class UI, class UO, bool GEN = false>
-__global__ __launch_bounds__(NTF) void k_md_su2_4d_tw_acc(UI Uin, UO Uout, T *__restrict__ P,
-                                                        FastGeo g, FusedCoef c,
-                                                        double *__restrict__ partials,
-                                                        int pstride) {
+__global__ __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void k_md_su2_4d_tw_acc(UI Uin, UO Uout, T *__restrict__ P, FastGeo g, FusedCoef c,
+                        double *__restrict__ partials, int pstride) {
   __shared__ double lds[3 * NTF / 64];
   const TwSite f = tw_site(g, blockIdx.x, threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
@@ -379,11 +381,11 @@ __global__ __launch_bounds__(NTF) void k_md_su2_4d_tw_acc(UI Uin, UO Uout, T *__
   }
   tw_partials(acc, lds, g.Tc, partials, pstride);
 }
+// (FP32 with W: 3 waves per SIMD; at 4 its 143 VGPRs would spill)
 template <typename T, bool W, int NT, class UI, class UO, bool GEN = false>
-__global__ __launch_bounds__(NTF) void k_md_su2_4d_twz_acc(UI Uin, UO Uout, T *__restrict__ P,
-                                                         FastGeo g, FusedCoef c,
-                                                         double *__restrict__ partials,
-                                                         int pstride) {
+__global__ __launch_bounds__(NTF) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !W ? 4 : 1)))
+void k_md_su2_4d_twz_acc(UI Uin, UO Uout, T *__restrict__ P, FastGeo g, FusedCoef c,
+                         double *__restrict__ partials, int pstride) {
   __shared__ double lds[3 * NTF / 64];
   const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
   double acc[3] = {0.0, 0.0, 0.0};
