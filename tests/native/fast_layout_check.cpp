@@ -208,7 +208,7 @@ int main() {
   const int geoms[][4] = {{32, 8, 8, 8},  {8, 8, 8, 8},     {16, 16, 16, 16}, {32, 32, 32, 32}, {24, 24, 24, 24},
                           {48, 48, 48, 48}, {16, 8, 8, 8},  {12, 8, 8, 8},    {4, 8, 8, 8},
                           {8, 4, 8, 16},  {8, 16, 16, 1},   {40, 16, 4, 4},   {8, 12, 12, 12},
-                          {64, 16, 16, 16}};
+                          {64, 16, 16, 16}, {40, 16, 8, 8}, {32, 8, 16, 16}};
   int fails = 0;
   for (const auto &L : geoms)
     for (int tb = 0; tb < 2; ++tb)

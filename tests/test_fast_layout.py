@@ -39,10 +39,13 @@ def test_fast_step_layout(tmp_path):
     for ext in ("48x48x48x48", "24x24x24x24", "32x8x8x8"):
         assert f"{ext} t-walk wmu: ok" in out.stdout
     # both walks with C t-chunks of Lt / C slices and 8 / C 3-volume parts
+    for ext in ("32x32x32x32", "48x48x48x48", "64x16x16x16", "40x16x8x8", "32x8x16x16"):
+        for wmu in ("", " wmu"):
+            for c in (2, 4):
+                assert f"{ext} t-walk{wmu} {c} chunks: ok" in out.stdout
     for ext in ("32x32x32x32", "48x48x48x48", "64x16x16x16"):
         for wmu in ("", " wmu"):
-            for c in (1, 2, 4):
-                assert f"{ext} t-walk{wmu} {c} chunks: ok" in out.stdout
+            assert f"{ext} t-walk{wmu} 1 chunks: ok" in out.stdout
     # the plaquette t-walk, with the +z lane exchange where 64 % Lz == 0
     for ext in ("32x32x32x32", "8x8x8x8", "32x8x8x8", "16x16x16x16"):
         assert f"{ext} plaquette t-walk z-exchange: ok" in out.stdout

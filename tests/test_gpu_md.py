@@ -758,7 +758,8 @@ print("NT_CHILD_OK", flush=True)
 """
 
 
-_TW_GEOMS = (("z8", (8, 8, 8, 32, 4)), ("z12", (8, 8, 12, 32, 4)), ("z8p4", (16, 8, 8, 32, 4)))
+_TW_GEOMS = (("z8", (8, 8, 8, 32, 4)), ("z12", (8, 8, 12, 32, 4)), ("z8p4", (16, 8, 8, 32, 4)),
+             ("t40", (16, 8, 8, 40, 4)), ("z16", (8, 16, 16, 32, 4)))
 _TW_CHILD = r"""
 import sys, numpy as np
 sys.path.insert(0, sys.argv[1])
@@ -787,8 +788,9 @@ def test_t_walk_step_parity(tmp_path, oracle):
     Lt / 2 slices (four 3-volume parts, where the 3-volume blocks divide by
     four: the (Lx, Ly, Lz, Lt) = (16, 8, 8, 32) lattice) against one chunk per
     XCD (SU2GPU_TWC=8): one OMF2(6) trajectory on (8, 8, 8, 32), (8, 8, 12, 32)
-    (SU2GPU_ZG=1: the FP32 general exchange, k_md_su2_4d_twz) and (16, 8, 8,
-    32) in both forms, in child processes because the switches are read once
+    (SU2GPU_ZG=1: the FP32 general exchange, k_md_su2_4d_twz), (16, 8, 8, 32),
+    (16, 8, 8, 40) (chunks of 20 and of 5 slices) and (8, 16, 16, 32) in both
+    forms, in child processes because the switches are read once
     per process.  FP32: links bit for bit (the same FMA chains), dH within
     1e-9 (SU2GPU_TW also selects the plaquette sum's kernel, k_plaq_su2_4d_tw
     or one thread per site, which add the same terms in another order); FP64:
