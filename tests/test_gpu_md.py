@@ -389,14 +389,18 @@ def test_fp32_trajectory_action_sums(S, oracle, dims, kind):
     assert np.abs((q.astype(np.float64) ** 2).sum(1) - 1.0).max() < 5e-7
 
 
-def test_device_momenta_statistics(S):
-    dims = (8, 8, 8, 8, 4)
-    with S.Lattice(S.SU2, dims, 2.4) as lat:
+@pytest.mark.parametrize("grp,dims,nm", [(1, (8, 8, 8, 8, 4), 3), (1, (16, 8, 8, 32, 4), 3),
+                                         (0, (64, 1, 1, 64, 2), 1)])
+def test_device_momenta_statistics(S, grp, dims, nm):
+    """K_old of md_update with device momenta is chi^2 / 2 with one degree of
+    freedom per momentum component: drawn by their own launch (8^4, U(1)
+    d = 2) or inside the first step's walk ((16, 8, 8, 32))."""
+    with S.Lattice(grp, dims, 2.4) as lat:
         lat.hotstart(SEED)
         lat.md_update(SEED, 6, 4, momenta=S.MOMENTA_DEVICE)
         r = lat.md_update(SEED + 1, 6, 4, momenta=S.MOMENTA_DEVICE)
         assert math.isfinite(r["deltaH"])
-        n = lat.nlinks * 3
+        n = lat.nlinks * nm
         K = r["kin_old"]
         assert abs(2 * K / n - 1.0) < 5 * math.sqrt(2.0 / n)
 
