@@ -449,15 +449,17 @@ def kernel_name(prec, L):
     lane exchange or in FP64 beyond 1 GiB) where the two link fields and the
     momenta exceed 256 MiB.
     The FP64 form steps on the half-field layout (QHalf in, QHalfW out); with
-    the lane exchange and >= 4 t-slices per XCD (FP32: within the Infinity
-    Cache) the step is the t-walk (k_md_su2_4d_tw<T, ZS, NT, in, out>)."""
+    >= 4 t-slices per XCD the step is a t-walk: FP64 with the lane exchange
+    k_md_su2_4d_tww<T, NT, in, out> (one direction per wave), FP32 with it
+    (within the Infinity Cache) k_md_su2_4d_tw<T, ZS, NT, in, out>, and with
+    the general exchange k_md_su2_4d_twz<T, NT, in, out>."""
     r = 8 if prec == 0 else 4
     big = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
     if prec == 0:
         zs = 1 if 64 % L == 0 else 0
         nt = (2 if 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3 else 1) if big else 0
         if zs == 1 and L % 8 == 0 and L // 8 >= 4:
-            return f"k_md_su2_4d_tw<double, 1, {nt}, su2k::QHalf, su2k::QHalfW>"
+            return f"k_md_su2_4d_tww<double, {nt}, su2k::QHalf, su2k::QHalfW>"
         if zs == 0 and L % 8 == 0 and L // 8 >= 4:
             return f"k_md_su2_4d_twz<double, {nt}, su2k::QHalf, su2k::QHalfW>"
         return f"k_md_su2_4d<double, true, false, 1, {zs}, false, {nt}, su2k::QHalf, su2k::QHalfW>"

@@ -515,13 +515,20 @@ static FastGeo tw_geo(const FastGeo &f, bool wmu) {
   if (c < 0) c = env_int("SU2GPU_TWC", 2);
   if (cz < 0) cz = env_int("SU2GPU_TWCZ", 8);
   FastGeo g = f;
-  if (wmu) c = cz;
-  const int lc = c == 1 ? 0 : c == 2 ? 1 : c == 4 ? 2 : 3;
+  const int cc = wmu ? cz : c;
+  const int lc = cc == 1 ? 0 : cc == 2 ? 1 : cc == 4 ? 2 : 3;
   const int C = 1 << lc, P = 8 / C;
   if (lc == 3 || f.Lt % C != 0 || (wmu ? f.nb64 : f.nb3) % P != 0 || f.Lt / C < 4) return g;
   g.tw_lc = lc;
   g.Tc = f.Lt / C;
   return g;
+}
+// SU2GPU_TWW = 0: the FP64 lane-exchange walk with one direction per block
+// (k_md_su2_4d_tw) instead of one per wave (k_md_su2_4d_tww)
+static int tww_pref() {
+  static int v = -1;
+  if (v < 0) v = env_int("SU2GPU_TWW", 1);
+  return v;
 }
 // SU2GPU_TWA = 0: the first and the last step of a trajectory (kinetic and
 // action partials, momentum-only last step) as one launch per link instead of
@@ -551,6 +558,23 @@ static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch 
   // bit of ~0.6 % of its values differs from this step's, the update's FMA
   // contraction, within the oracle parity either way)
   if constexpr (W && !A) {
+    // one direction per wave (k_md_su2_4d_tww; SU2GPU_TWW=0: one per block,
+    // k_md_su2_4d_tw): 32^4 138.3-138.6 against 139.6-139.7 us (alternating
+    // runs on one box, the mapping of the general-exchange walk)
+    if (zs && f.tblock && f.Tc >= 4 && tw_pref() && tww_pref()) {
+      const FastGeo g = tw_geo(f, true);
+      const dim3 gt(tw_blocks_w(g)), bt(NTF);
+      if (nt == 2)
+        hipLaunchKernelGGL((k_md_su2_4d_tww<double, 2, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, g, L.c);
+      else if (nt == 1)
+        hipLaunchKernelGGL((k_md_su2_4d_tww<double, 1, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, g, L.c);
+      else
+        hipLaunchKernelGGL((k_md_su2_4d_tww<double, 0, UI, QHalfW>), gt, bt, 0, st, uin, uout,
+                           (double *)L.P, g, L.c);
+      return;
+    }
     if (zs && f.tblock && f.Tc >= 4 && tw_pref()) {
       const FastGeo g = tw_geo(f, false);
       const dim3 gt(tw_blocks(g)), bt(NTF);

@@ -346,6 +346,23 @@ void k_md_su2_4d_twz(UI Uin, UO Uout, T *__restrict__ P, FastGeo g, FusedCoef c)
   }
 }
 
+// the t-walk with the whole-row +-z exchange and one direction per wave (the
+// mapping of k_md_su2_4d_twz: a block holds 64 sites, its four waves their
+// four directions, so the links the directions share are requested by waves
+// of one CU); the FP64 step where 64 % Lz == 0
+template <typename T, int NT, class UI = const quat<T> *, class UO = quat<T> *>
+__global__ __launch_bounds__(NTF) void k_md_su2_4d_tww(UI Uin, UO Uout, T *__restrict__ P,
+                                                     FastGeo g, FusedCoef c) {
+  const TwSite f = tw_site_w(g, blockIdx.x, threadIdx.x);
+  double acc[3];
+  switch (__builtin_amdgcn_readfirstlane(f.mu)) {
+    case 0: md_su2_tw_body<T, 0, 1, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    case 1: md_su2_tw_body<T, 1, 1, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    case 2: md_su2_tw_body<T, 2, 1, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+    default: md_su2_tw_body<T, 3, 1, NT, UI, UO>(Uin, Uout, P, g, c, f, acc); break;
+  }
+}
+
 // The block's partial sums of a t-walk with ACC: one block does the work of
 // Tc blocks of the one-launch step (Tc slices of NTF links), so its sums go
 // to entry Tc b of each row and entries Tc b + 1 .. Tc b + Tc - 1 are written

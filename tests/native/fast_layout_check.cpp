@@ -79,6 +79,17 @@ static int check(const int L[4], bool tblock, bool xwalk, bool wmu = false) {
           printf("FAIL t-walk range b=%u tid=%u mu=%d o3=%d t0=%d\n", b, tid, f.mu, f.o3, f.t0);
           return 1;
         }
+        // the whole-row exchange (64 % Lz == 0) on either mapping: lane l's
+        // z is l mod Lz, and the lane the exchange reads holds the +z site
+        if (64 % L[3] == 0) {
+          const uint32_t lane = tid & 63u, zl = lane & (uint32_t)(L[3] - 1);
+          const uint32_t nb_lane = zl == (uint32_t)(L[3] - 1) ? lane - (L[3] - 1) : lane + 1;
+          const TwSite n = wmu ? tw_site_w(gw, b, tid - lane + nb_lane) : tw_site(gw, b, tid - lane + nb_lane);
+          if ((uint32_t)(f.o3 % L[3]) != zl || n.o3 != f.o3 + f.up[3] || n.mu != f.mu) {
+            printf("FAIL t-walk +z lane of b=%u tid=%u\n", b, tid);
+            return 1;
+          }
+        }
         for (int kt = 0; kt < gw.Tc; ++kt) {
           const int t = f.t0 + kt, s = t * g.vol3 + f.o3;
           if (seen_t[(size_t)f.mu * V + s]++) {

@@ -791,13 +791,16 @@ def test_t_walk_step_parity(tmp_path, oracle):
     one-launch-per-link step (SU2GPU_TW=0), and its default two t-chunks of
     Lt / 2 slices (four 3-volume parts, where the 3-volume blocks divide by
     four: the (Lx, Ly, Lz, Lt) = (16, 8, 8, 32) lattice) against one chunk per
-    XCD (SU2GPU_TWC=8): one OMF2(6) trajectory on (8, 8, 8, 32), (8, 8, 12, 32)
+    XCD (SU2GPU_TWC=8), and the FP64 walk with one direction per wave
+    (k_md_su2_4d_tww, the default) against one per block (SU2GPU_TWW=0,
+    k_md_su2_4d_tw): one OMF2(6) trajectory on (8, 8, 8, 32), (8, 8, 12, 32)
     (SU2GPU_ZG=1: the FP32 general exchange, k_md_su2_4d_twz), (16, 8, 8, 32),
     (16, 8, 8, 40) (chunks of 20 and of 5 slices) and (8, 16, 16, 32) in both
     forms, in child processes because the switches are read once
     per process.  FP32: links bit for bit (the same FMA chains), dH within
-    1e-9 (SU2GPU_TW also selects the plaquette sum's kernel, k_plaq_su2_4d_tw
-    or one thread per site, which add the same terms in another order); FP64:
+    1e-9 (the kinetic and action partial sums of the first and last steps, and
+    with SU2GPU_TW=0 the plaquette sums, add the same terms in another order);
+    FP64:
     the last bit of some values may differ (the compiler contracts the
     update's products per code path), so each against the oracle's trajectory,
     1e-11 per link."""
@@ -806,11 +809,11 @@ def test_t_walk_step_parity(tmp_path, oracle):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     keys = [t + n for t, _ in _TW_GEOMS for n in ("fp32", "fp64")]
     cases = {"tw": {"SU2GPU_TW": "1"}, "tw8": {"SU2GPU_TW": "1", "SU2GPU_TWC": "8"},
-             "one": {"SU2GPU_TW": "0"}}
+             "blk": {"SU2GPU_TW": "1", "SU2GPU_TWW": "0"}, "one": {"SU2GPU_TW": "0"}}
     outs = {}
     for case, env in cases.items():
         base = str(tmp_path / f"links_{case}_")
-        e = {k: v for k, v in os.environ.items() if k not in ("SU2GPU_TW", "SU2GPU_TWC")}
+        e = {k: v for k, v in os.environ.items() if k not in ("SU2GPU_TW", "SU2GPU_TWC", "SU2GPU_TWW")}
         e.update(SU2GPU_ZG="1", **env)
         out = subprocess.run([sys.executable, "-c", _TW_CHILD, root, base], env=e,
                              capture_output=True, text=True, timeout=300)
@@ -819,10 +822,10 @@ def test_t_walk_step_parity(tmp_path, oracle):
         dh = {ln.split()[1]: ln.split()[2] for ln in out.stdout.splitlines() if ln.startswith("TW_CHILD_OK")}
         outs[case] = {k: (np.load(base + k + ".npy"), dh[k]) for k in keys}
     for tag, dims in _TW_GEOMS:
-        for case in ("tw8", "one"):
+        for case in ("tw8", "blk", "one"):
             np.testing.assert_array_equal(outs["tw"][tag + "fp32"][0], outs[case][tag + "fp32"][0])
-        assert outs["tw"][tag + "fp32"][1] == outs["tw8"][tag + "fp32"][1]
-        assert abs(float(outs["tw"][tag + "fp32"][1]) - float(outs["one"][tag + "fp32"][1])) < 1e-9
+        for case in ("tw8", "blk", "one"):
+            assert abs(float(outs["tw"][tag + "fp32"][1]) - float(outs[case][tag + "fp32"][1])) < 1e-9
         U = oracle.hotstart(1, dims, 91, 0.5)
         P = oracle.initnormal(1, dims, 92)
         Uw, _ = oracle.integrate(1, dims, U, P, 2.4, 6, 6)
