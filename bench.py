@@ -450,16 +450,19 @@ def kernel_name(prec, L):
     momenta exceed 256 MiB.
     The FP64 form steps on the half-field layout (QHalf in, QHalfW out); with
     >= 4 t-slices per XCD the step is a t-walk: FP64 with the lane exchange
-    k_md_su2_4d_tww<T, NT, in, out> (one direction per wave), FP32 with it
-    (within the Infinity Cache) k_md_su2_4d_tw<T, ZS, NT, in, out>, and with
-    the general exchange k_md_su2_4d_twz<T, NT, in, out>."""
+    k_md_su2_4d_tww<T, NT, in, out> (one direction per wave; beyond 1 GiB of
+    fields k_md_su2_4d_tw), FP32 with it (within the Infinity Cache)
+    k_md_su2_4d_tw<T, ZS, NT, in, out>, and with the general exchange
+    k_md_su2_4d_twz<T, NT, in, out>."""
     r = 8 if prec == 0 else 4
     big = 4 * L ** 4 * (2 * 4 * r + 3 * r) > 256 * 1024 * 1024
     if prec == 0:
         zs = 1 if 64 % L == 0 else 0
         nt = (2 if 4 * L ** 4 * (2 * 4 * r + 3 * r) > 1024 ** 3 else 1) if big else 0
         if zs == 1 and L % 8 == 0 and L // 8 >= 4:
-            return f"k_md_su2_4d_tww<double, {nt}, su2k::QHalf, su2k::QHalfW>"
+            if nt < 2:
+                return f"k_md_su2_4d_tww<double, {nt}, su2k::QHalf, su2k::QHalfW>"
+            return f"k_md_su2_4d_tw<double, 1, {nt}, su2k::QHalf, su2k::QHalfW>"
         if zs == 0 and L % 8 == 0 and L // 8 >= 4:
             return f"k_md_su2_4d_twz<double, {nt}, su2k::QHalf, su2k::QHalfW>"
         return f"k_md_su2_4d<double, true, false, 1, {zs}, false, {nt}, su2k::QHalf, su2k::QHalfW>"

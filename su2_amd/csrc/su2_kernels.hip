@@ -558,10 +558,12 @@ static void launch_half(bool zs, int nt, UI uin, QHalfW uout, const FusedLaunch 
   // bit of ~0.6 % of its values differs from this step's, the update's FMA
   // contraction, within the oracle parity either way)
   if constexpr (W && !A) {
-    // one direction per wave (k_md_su2_4d_tww; SU2GPU_TWW=0: one per block,
-    // k_md_su2_4d_tw): 32^4 138.3-138.6 against 139.6-139.7 us (alternating
-    // runs on one box, the mapping of the general-exchange walk)
-    if (zs && f.tblock && f.Tc >= 4 && tw_pref() && tww_pref()) {
+    // one direction per wave (k_md_su2_4d_tww, the mapping of the
+    // general-exchange walk; SU2GPU_TWW=0: one per block, k_md_su2_4d_tw)
+    // below 1 GiB of fields: 32^4 138.3-138.6 against 139.6-139.7 us
+    // (alternating runs on one box); beyond it one per block in two chunks
+    // (64^4: 2462 against 2511-2513 us)
+    if (zs && f.tblock && f.Tc >= 4 && tw_pref() && tww_pref() && nt < 2) {
       const FastGeo g = tw_geo(f, true);
       const dim3 gt(tw_blocks_w(g)), bt(NTF);
       if (nt == 2)
