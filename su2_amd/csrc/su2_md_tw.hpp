@@ -3,9 +3,9 @@
 // update_gauge.hh:11-14): the arithmetic of md_su2_core (su2_md_fast.hpp), bit
 // for bit, with fewer link loads.
 //
-// A thread keeps one 3-volume site and one direction mu and walks the t-slices
-// of its XCD's chunk [xcd Tc, (xcd + 1) Tc) upwards.  The staple links that the
-// link at t + 1 shares with the link at t stay in registers:
+// A thread keeps one 3-volume site and one direction mu and walks the Tc
+// t-slices of a chunk upwards.  The staple links that the link at t + 1 shares
+// with the link at t stay in registers:
 //   * mu != t, plane t: U_t(s+mu-t), U_mu(s-t), U_t(s-t) of the link at t + 1
 //     are U_t(s+mu), U_mu(s), U_t(s) of the link at t, and its own link
 //     U_mu(s+t) is the link at t's U_mu(s+t): 2 loads instead of 6 + 1;
@@ -15,16 +15,22 @@
 // instead of 14-15; the chunk's first slice loads what the slice below would
 // have passed on (its t-1 neighbours come from the neighbouring XCD's chunk).
 //
-// Launch: 32 nb3 blocks of NTF threads (nb3 = vol3 / NTF); hardware block b
-// runs on XCD b % 8, which owns the chunk; local = b / 8 = 4 q + mu, site
-// o3 = q NTF + tid.  Needs the t-blocked layout (Lt % 8 == 0).
+// Chunks: C = 2^tw_lc t-chunks of Tc = Lt / C slices; hardware block b runs on
+// XCD k = b % 8, which walks chunk k mod C of the 3-volume part k / C (8 / C
+// parts; C = 8: one chunk of Lt / 8 slices per XCD and the whole 3-volume).
+// Needs the t-blocked layout (Lt % 8 == 0).  k_md_su2_4d_tw: one direction per
+// block, 32 nb3 / (8 / C) blocks of NTF threads (nb3 = vol3 / NTF), local =
+// b / 8 = 4 q + mu, site o3 = (part nq + q) NTF + tid.
 //
-// k_md_su2_4d_twz: the same walk with the general +-z exchange (ZS = 2, any
-// Lz: the neighbour of lane l is lane l +- 1, or l -+ (Lz - 1) at a row's wrap,
-// where that lane is in the wave; the lanes whose neighbour is in another
-// wave load it in the same memory round trip) and one direction per wave: a
-// block holds 64 sites (o3 = q 64 + lane) and its wave w the direction w, so
-// the four directions of the same sites walk t together on one CU.
+// One direction per wave (tw_site_w): a block holds 64 sites (o3 = (part nq +
+// q) 64 + lane) and its wave w the direction w, so the four directions of the
+// same sites walk t together on one CU.  k_md_su2_4d_twz takes it with the
+// general +-z exchange (ZS = 2, any Lz: the neighbour of lane l is lane l +- 1,
+// or l -+ (Lz - 1) at a row's wrap, where that lane is in the wave; the lanes
+// whose neighbour is in another wave load it in the same memory round trip),
+// k_md_su2_4d_tww with the whole-row exchange (ZS = 1).  The _acc kernels are
+// the first and last steps of a trajectory on either mapping; k_plaq_su2_4d_tw
+// is the plaquette sum walking t.
 #pragma once
 #include "su2_md_fast.hpp"
 
